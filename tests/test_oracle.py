@@ -1,0 +1,476 @@
+"""Pins for the FP64 oracle: closed forms, brute force, invariants and finite differences.
+
+The reference ships no golden vectors or known-answer tests; every check here
+comes from the mathematics, chosen so that a plausible slip (a dropped term,
+a wrong sign, index or transposition) fails at least one of them.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from cusoap_amd import systems
+from oracle import soap_oracle as so
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+# ----------------------------------------------------------------------------
+# Spherical / solid harmonics
+# ----------------------------------------------------------------------------
+
+def _table_l3(u):
+    """Real spherical harmonics for l<=3 written out from the standard table."""
+    x, y, z = u[:, 0], u[:, 1], u[:, 2]
+    pi = math.pi
+    cols = [
+        np.full_like(x, 0.5 * math.sqrt(1 / pi)),
+        math.sqrt(3 / (4 * pi)) * y, math.sqrt(3 / (4 * pi)) * z, math.sqrt(3 / (4 * pi)) * x,
+        0.5 * math.sqrt(15 / pi) * x * y, 0.5 * math.sqrt(15 / pi) * y * z,
+        0.25 * math.sqrt(5 / pi) * (3 * z * z - 1), 0.5 * math.sqrt(15 / pi) * x * z,
+        0.25 * math.sqrt(15 / pi) * (x * x - y * y),
+        0.25 * math.sqrt(35 / (2 * pi)) * y * (3 * x * x - y * y),
+        0.5 * math.sqrt(105 / pi) * x * y * z,
+        0.25 * math.sqrt(21 / (2 * pi)) * y * (5 * z * z - 1),
+        0.25 * math.sqrt(7 / pi) * (5 * z ** 3 - 3 * z),
+        0.25 * math.sqrt(21 / (2 * pi)) * x * (5 * z * z - 1),
+        0.25 * math.sqrt(105 / pi) * (x * x - y * y) * z,
+        0.25 * math.sqrt(35 / (2 * pi)) * x * (x * x - 3 * y * y),
+    ]
+    return np.stack(cols, axis=1)
+
+
+def test_sh_closed_forms_l3():
+    u = _unit(np.random.default_rng(1), 200)
+    np.testing.assert_allclose(so.spherical_harmonics(3, u), _table_l3(u), rtol=0, atol=1e-14)
+
+
+def test_solid_harmonics_homogeneous():
+    rng = np.random.default_rng(2)
+    u = _unit(rng, 50)
+    r = rng.uniform(0.1, 5.0, size=50)
+    Ys = so.solid_harmonics(7, u * r[:, None])
+    Yu = so.spherical_harmonics(7, u)
+    lofm = np.array([l for l in range(8) for _ in range(2 * l + 1)])
+    np.testing.assert_allclose(Ys, Yu * r[:, None] ** lofm[None, :], rtol=1e-12, atol=1e-14)
+
+
+def test_sh_orthonormal_quadrature():
+    lmax = 8
+    x, w = np.polynomial.legendre.leggauss(24)
+    nphi = 40
+    phi = 2 * np.pi * np.arange(nphi) / nphi
+    ct, ph = np.meshgrid(x, phi, indexing="ij")
+    st = np.sqrt(1 - ct ** 2)
+    u = np.stack([st * np.cos(ph), st * np.sin(ph), ct], -1).reshape(-1, 3)
+    wt = (w[:, None] * np.full(nphi, 2 * np.pi / nphi)[None, :]).reshape(-1)
+    Y = so.spherical_harmonics(lmax, u)
+    G = (Y * wt[:, None]).T @ Y
+    np.testing.assert_allclose(G, np.eye(G.shape[0]), atol=1e-12)
+
+
+def test_sh_addition_theorem():
+    rng = np.random.default_rng(3)
+    u, v = _unit(rng, 30), _unit(rng, 30)
+    lmax = 9
+    Yu, Yv = so.spherical_harmonics(lmax, u), so.spherical_harmonics(lmax, v)
+    cosg = (u * v).sum(1)
+    for l in range(lmax + 1):
+        sl = slice(l * l, (l + 1) ** 2)
+        lhs = (Yu[:, sl] * Yv[:, sl]).sum(1)
+        rhs = (2 * l + 1) / (4 * np.pi) * special.eval_legendre(l, cosg)
+        np.testing.assert_allclose(lhs, rhs, atol=1e-12)
+
+
+def test_solid_harmonic_gradients_fd():
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=(20, 3)) * 1.5
+    _, G = so.solid_harmonics(6, p, grad=True)
+    h = 1e-5
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = h
+        fd = (so.solid_harmonics(6, p + e) - so.solid_harmonics(6, p - e)) / (2 * h)
+        np.testing.assert_allclose(G[:, d, :], fd, rtol=1e-7, atol=1e-7)
+    # at the origin only l=1 has a gradient: sqrt(3/4pi) * (e_y, e_z, e_x)
+    _, G0 = so.solid_harmonics(3, np.zeros((1, 3)), grad=True)
+    k = math.sqrt(3 / (4 * math.pi))
+    expect = np.zeros((3, 16))
+    expect[1, 1] = expect[2, 2] = expect[0, 3] = k
+    np.testing.assert_allclose(G0[0], expect, atol=1e-15)
+
+
+# ----------------------------------------------------------------------------
+# Radial bases
+# ----------------------------------------------------------------------------
+
+def test_gto_lowdin_identities():
+    alphas, betas = so.gto_basis(6.0, 5, 4)
+    for l in range(5):
+        S = so.gto_overlap(alphas[l], l)
+        b = betas[l]
+        np.testing.assert_allclose(b, b.T, atol=1e-10 * np.abs(b).max())
+        assert np.linalg.eigvalsh(b).min() > 0
+        np.testing.assert_allclose(b @ S @ b, np.eye(5), atol=1e-8)
+
+
+def test_gto_alpha_definition():
+    alphas, _ = so.gto_basis(6.0, 4, 3)
+    a = np.linspace(1, 6, 4)
+    for l in range(4):
+        # primitive r^l exp(-alpha r^2) decays to 1e-3 at r = a
+        np.testing.assert_allclose(a ** l * np.exp(-alphas[l] * a * a), 1e-3, rtol=1e-12)
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 4, 6])
+def test_gto_projection_closed_form(l):
+    """pref r^l exp(-gamma r^2) = 4 pi e^{-eta r^2} int r'^{l+2} e^{-p r'^2} i_l(2 eta r r') dr'."""
+    sigma = 0.5
+    eta = 1 / (2 * sigma ** 2)
+    for alpha in (0.2, 1.3):
+        p = alpha + eta
+        gamma = alpha * eta / p
+        pref = math.pi ** 1.5 * (eta / p) ** l * p ** -1.5
+        for r in (0.3, 1.7, 3.2):
+            f = lambda rp: rp ** (l + 2) * math.exp(-p * rp * rp + 2 * eta * r * rp - eta * r * r) \
+                * special.spherical_in(l, 2 * eta * r * rp) * math.exp(-2 * eta * r * rp)
+            val, _ = integrate.quad(f, 0, 30, limit=400, epsabs=0, epsrel=1e-12)
+            np.testing.assert_allclose(4 * math.pi * val, pref * r ** l * math.exp(-gamma * r * r), rtol=1e-9)
+
+
+def test_gto_radial_orthonormal():
+    o = so.OracleSOAP(r_cut=5.0, n_max=5, l_max=3, sigma=0.4, species=[1])
+    x, w = np.polynomial.legendre.leggauss(200)
+    R = 25.0
+    r = R * 0.5 * (x + 1)
+    wr = R * 0.5 * w
+    for l in range(4):
+        g = np.stack([(o.betas[l][n] [None, :] * (r[:, None] ** l) * np.exp(-o.alphas[l][None, :] * (r * r)[:, None])).sum(1)
+                      for n in range(5)])
+        M = (g * (r * r * wr)[None, :]) @ g.T
+        np.testing.assert_allclose(M, np.eye(5), atol=1e-7)
+
+
+def test_poly_basis_orthonormal():
+    rx, wr, gss, _ = so.poly_basis(5.0, 5, 100)
+    M = (gss * (rx * rx * wr)[None, :]) @ gss.T
+    np.testing.assert_allclose(M, np.eye(5), atol=1e-6)
+
+
+def test_poly_radial_against_scipy_bessel():
+    o = so.OracleSOAP(r_cut=5.0, n_max=4, l_max=5, sigma=0.5, rbf="polynomial", species=[1])
+    r = np.array([0.5, 1.3, 2.2, 4.0, 5.5])
+    I, J = o.poly_I_J(r, want_J=True)
+    eta = o.eta
+    for l in range(6):
+        t = 2 * eta * o.rx[None, :] * r[:, None]
+        ker = 4 * np.pi * o.rx[None] ** 2 * o.wr[None] * np.exp(-eta * (o.rx[None] ** 2 + r[:, None] ** 2))
+        Iref = (ker * special.spherical_in(l, t)) @ o.gss.T
+        Jref = (ker * o.rx[None] * special.spherical_in(l + 1, t)) @ o.gss.T
+        scale = np.abs(Iref).max()
+        np.testing.assert_allclose(I[:, :, l], Iref, atol=1e-9 * scale)
+        np.testing.assert_allclose(J[:, :, l], Jref, atol=1e-9 * np.abs(Jref).max())
+
+
+def test_poly_B_limits():
+    o = so.OracleSOAP(r_cut=5.0, n_max=4, l_max=4, sigma=0.5, rbf="polynomial", species=[1])
+    B0, Bh0 = o.poly_B_Bhat_r0()
+    # B = I / r^l and Bhat = 2 eta (J - r I) / r^(l+1) near r = 0, with scipy's
+    # Bessel functions (accurate at small argument, unlike the upward recurrence)
+    eta = o.eta
+
+    def BB(r, l):
+        t = 2 * eta * o.rx * r
+        ker = 4 * np.pi * o.rx ** 2 * o.wr * np.exp(-eta * (o.rx ** 2 + r * r))
+        I = (ker * special.spherical_in(l, t)) @ o.gss.T
+        J = (ker * o.rx * special.spherical_in(l + 1, t)) @ o.gss.T
+        return I / r ** l, 2 * eta * (J - r * I) / r ** (l + 1)
+
+    for l in range(5):
+        (b1, h1), (b2, h2) = BB(1e-3, l), BB(2e-3, l)
+        # Richardson extrapolation of the even functions B(r), Bhat(r) to r = 0
+        np.testing.assert_allclose((4 * b1 - b2) / 3, B0[:, l], rtol=1e-7, atol=1e-10)
+        np.testing.assert_allclose((4 * h1 - h2) / 3, Bh0[:, l], rtol=1e-5,
+                                   atol=1e-7 * np.abs(Bh0).max())
+    # the closed-form path overrides r < 1e-4 with these limits
+    B, Bh = o.radial_B_Bhat(np.array([0.0, 5e-5]))
+    np.testing.assert_array_equal(B[0], B0)
+    np.testing.assert_array_equal(Bh[1], Bh0)
+
+
+# ----------------------------------------------------------------------------
+# Weighting
+# ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("wd", [dict(function="pow", r0=1.5, c=1.0, d=1.0, m=2.0, w0=0.5),
+                                dict(function="exp", r0=0.7, c=2.0, d=0.5, w0=1.0),
+                                dict(function="poly", r0=4.0, c=1.0, m=2.0, w0=1.0)])
+def test_weighting_closed_forms(wd):
+    W = so.Weighting(**wd)
+    r = np.array([0.0, 0.5, 1.0, 3.0, 3.9])
+    w = W(r)
+    fn = wd["function"]
+    if fn == "pow":
+        ref = wd["c"] / (wd["d"] + (r / wd["r0"]) ** wd["m"])
+    elif fn == "exp":
+        ref = wd["c"] / (wd["d"] + np.exp(-r / wd["r0"]))
+    else:
+        x = r / wd["r0"]
+        ref = wd["c"] * (1 + 2 * x ** 3 - 3 * x ** 2) ** wd["m"]
+    ref[0] = wd["w0"]
+    np.testing.assert_allclose(w, ref, rtol=1e-14)
+    rr = np.array([0.4, 1.1, 2.5, 3.3])
+    h = 1e-6
+    np.testing.assert_allclose(W.deriv(rr), (W(rr + h) - W(rr - h)) / (2 * h), rtol=1e-6)
+
+
+def test_weighting_infer_rcut():
+    assert so.Weighting(function="poly", r0=3.0).infer_rcut() == 3.0
+    r = so.Weighting(function="pow", r0=1.0, c=1.0, d=1.0, m=2.0).infer_rcut(1e-2)
+    np.testing.assert_allclose(1.0 / (1.0 + r ** 2), 1e-2)
+    with pytest.raises(ValueError):
+        so.Weighting(function=None).infer_rcut()
+
+
+# ----------------------------------------------------------------------------
+# Neighbour search
+# ----------------------------------------------------------------------------
+
+def test_neighbour_list_matches_brute_force():
+    s = systems.water_cluster_300(seed=5)
+    pos = s["positions"]
+    cutoff = 6.0 + 0.5 * math.sqrt(-2 * math.log(1e-3))
+    ci, ni = so.radius_edges(pos, pos, cutoff)
+    d2 = ((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+    a, b = np.nonzero(d2 <= cutoff * cutoff)
+    assert so.cutoff_band_pairs(pos, pos, cutoff) == 0
+    np.testing.assert_array_equal(ci, a)
+    np.testing.assert_array_equal(ni, b)
+    # order: centre ascending, neighbour ascending within a centre
+    key = ci * pos.shape[0] + ni
+    assert np.all(np.diff(key) > 0)
+
+
+def test_periodic_images_against_wide_enumeration():
+    s = systems.water((3, 3, 3), spacing=3.1, seed=2, cell_length=9.3)
+    pos, cell = s["positions"], s["cell"]
+    cutoff = 7.0
+    pos_ext, _, shift = so.extend_periodic(pos, s["atomic_numbers"], cell, s["pbc"], cutoff)
+    ci, ni = so.radius_edges(pos, pos_ext, cutoff)
+    counts = np.bincount(ci, minlength=pos.shape[0])
+    rng = np.arange(-3, 4)
+    ox, oy, oz = np.meshgrid(rng, rng, rng, indexing="ij")
+    wide = np.stack([ox.ravel(), oy.ravel(), oz.ravel()], 1) @ cell
+    allp = (pos[None] + wide[:, None]).reshape(-1, 3)
+    d2 = ((pos[:, None] - allp[None]) ** 2).sum(-1)
+    np.testing.assert_array_equal(counts, (d2 <= cutoff ** 2).sum(1))
+    # the ext index is image*N + atom with images in meshgrid-ij order of -n..n
+    n_img = shift.shape[0]
+    assert n_img == 27
+    np.testing.assert_allclose(pos_ext[13 * pos.shape[0]:14 * pos.shape[0]], pos)
+
+
+def test_pbc_offsets_non_orthogonal():
+    cell = np.array([[5.0, 0, 0], [2.5, 4.0, 0], [0.3, 0.2, 6.0]])
+    offs = so.pbc_offsets(cell, np.array([True, True, False]), 7.0)
+    h = so.perpendicular_lengths(cell)
+    n = [math.ceil(7.0 / (h[i] + 1e-12)) for i in range(2)]
+    assert offs.shape[0] == (2 * n[0] + 1) * (2 * n[1] + 1)
+    assert np.all(offs[:, 2] == 0)
+    np.testing.assert_allclose(h[2], abs(np.linalg.det(cell)) / np.linalg.norm(np.cross(cell[0], cell[1])))
+
+
+# ----------------------------------------------------------------------------
+# Descriptor assembly
+# ----------------------------------------------------------------------------
+
+def _small_water(seed=0, nmol=4):
+    return systems.water((2, 2, 1), seed=seed, n_mol=nmol)
+
+
+def test_feature_counts_and_locations():
+    for cross in (True, False):
+        o = so.OracleSOAP(r_cut=4.0, n_max=3, l_max=2, sigma=0.5, species=[1, 6, 8], crossover=cross)
+        ft = o.feature_table()
+        assert ft.shape[0] == o.n_power_features()
+        if cross:
+            assert o.n_power_features() == (9 * 10 // 2) * 3
+            # get_location blocks tile the vector in (j<=jd) order
+            starts = [o.get_location((a, b)).start for a, b in [(1, 1), (1, 6), (1, 8), (6, 6), (6, 8), (8, 8)]]
+            assert starts == sorted(starts) and starts[0] == 0
+            sl = o.get_location((8, 6))
+            assert np.all(ft[sl, 0] == 1) and np.all(ft[sl, 1] == 2)
+        else:
+            assert o.n_power_features() == 3 * 3 * 6
+
+
+def test_power_spectrum_direct_and_invariances():
+    s = _small_water(1)
+    o = so.OracleSOAP(r_cut=4.0, n_max=3, l_max=3, sigma=0.5, species=[1, 8])
+    d, c = o.create(s, return_coeffs=True)
+    ft = o.feature_table()
+    for f in (0, 5, 17, len(ft) - 1):
+        j, jd, a, b, l = ft[f]
+        ref = o.prefactor(l) * (c[:, j, a, l * l:(l + 1) ** 2] * c[:, jd, b, l * l:(l + 1) ** 2]).sum(1)
+        np.testing.assert_allclose(d[:, f], ref, rtol=1e-13)
+    # rotation invariance of the power spectrum
+    R = systems._quat_to_rot(np.random.default_rng(9).normal(size=(1, 4)))[0]
+    s2 = dict(s, positions=s["positions"] @ R.T)
+    np.testing.assert_allclose(o.create(s2), d, atol=1e-11 * np.abs(d).max())
+    # permutation of atoms permutes the rows
+    perm = np.random.default_rng(3).permutation(s["positions"].shape[0])
+    s3 = {"positions": s["positions"][perm], "atomic_numbers": s["atomic_numbers"][perm]}
+    np.testing.assert_allclose(o.create(s3), d[perm], atol=1e-12 * np.abs(d).max())
+
+
+def test_average_modes_identical_rows_and_cc_layout():
+    s = _small_water(2)
+    outs = [so.OracleSOAP(r_cut=4.0, n_max=2, l_max=2, sigma=0.5, species=[1, 8], average=a).create(s)
+            for a in ("off", "inner", "outer")]
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])
+    o = so.OracleSOAP(r_cut=4.0, n_max=2, l_max=2, sigma=0.5, species=[1, 8], average="cc")
+    cc, c = o.create(s, return_coeffs=True)
+    cc = cc.reshape(c.shape)
+    np.testing.assert_array_equal(cc[..., 0], c[..., 0])
+    np.testing.assert_array_equal(cc[..., 1:4], c[..., [2, 3, 1]])       # (z, x, y)
+    np.testing.assert_allclose(cc[..., 4:], c[..., 4:] * math.pi ** -1.5)
+
+
+def test_self_term_is_r0_limit():
+    """Dropping the r=0 edge and adding the self term equals keeping the edge."""
+    s = _small_water(3)
+    for rbf in ("gto", "polynomial"):
+        o = so.OracleSOAP(r_cut=4.0, n_max=3, l_max=3, sigma=0.5, species=[1, 8], rbf=rbf)
+        pos, Z = s["positions"], s["atomic_numbers"]
+        cx, cidx = o.prepare_centers(pos, None)
+        ci, ni, rv, sp = o.neighbour_list(pos, Z, cx, drop_self=True)
+        c1 = o.coefficients_create(ci, sp, rv, cx.shape[0])
+        o.add_self_terms(c1, cidx, Z)
+        ci, ni, rv, sp = o.neighbour_list(pos, Z, cx, drop_self=False)
+        c2 = o.coefficients_solid(ci, sp, rv, cx.shape[0])
+        np.testing.assert_allclose(c1, c2, atol=1e-11 * np.abs(c2).max())
+        if rbf == "gto":
+            B, _ = o.radial_B_Bhat(np.zeros(1))
+            np.testing.assert_allclose(o.self_l0, B[0, :, 0] * so.Y00, rtol=1e-13)
+            np.testing.assert_allclose(o.self_l1, B[0, :, 1], rtol=1e-13)
+
+
+def test_coordinate_centers_snap_to_atoms():
+    s = _small_water(4)
+    o = so.OracleSOAP(r_cut=4.0, n_max=2, l_max=2, sigma=0.5, species=[1, 8])
+    d_idx = o.create(s, centers=[0, 5])
+    d_xyz = o.create(s, centers=[s["positions"][0].tolist(), s["positions"][5].tolist()])
+    np.testing.assert_array_equal(d_idx, d_xyz)
+
+
+# ----------------------------------------------------------------------------
+# Jacobians against finite differences
+# ----------------------------------------------------------------------------
+
+def _fd_check(o, s, centers=None, include=None, attach=False, tol=2e-6):
+    J = o.derivatives(s, centers=centers, include=include, attach=attach, return_descriptor=False)
+    F = o.derivatives_numerical(s, centers=centers, include=include, attach=attach)
+    scale = np.abs(F).max()
+    assert scale > 0
+    err = np.abs(J - F).max() / scale
+    assert err < tol, err
+    return J
+
+
+@pytest.mark.parametrize("average", ["off", "cc"])
+def test_jacobian_gto_fd(average):
+    s = _small_water(5, nmol=3)
+    o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=3, sigma=0.5, species=[1, 8], average=average)
+    _fd_check(o, s, centers=[0, 4, 7], include=[8, 0, 3, 4])
+
+
+def test_jacobian_poly_fd():
+    s = _small_water(6, nmol=3)
+    o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=2, sigma=0.5, species=[1, 8], rbf="polynomial")
+    _fd_check(o, s, centers=[1, 2], tol=1e-5)
+
+
+@pytest.mark.parametrize("wd", [dict(function="pow", r0=1.0, c=1.0, d=1.0, m=2.0, w0=1.0),
+                                dict(function="exp", r0=0.8, c=1.5, d=0.5, w0=1.0)])
+def test_jacobian_weighting_fd(wd):
+    # w0 equals w(0+) here, so the weight is continuous at the self edge and
+    # finite differences see the same l=1 self-edge slope as the analytic form
+    s = _small_water(7, nmol=3)
+    o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=2, sigma=0.5, species=[1, 8], weighting=wd)
+    if wd["function"] == "exp":
+        # exp has w'(0) != 0: w(|x|) has a kink at a self edge, where central
+        # differences are only first order; use off-atom coordinate centers
+        centers = [[0.4, 0.3, -0.2], [1.9, 0.8, 0.5]]
+    else:
+        centers = [0, 3, 8]
+    _fd_check(o, s, centers=centers)
+
+
+def test_jacobian_weighting_w0_scales_self_edge_only():
+    """The analytic self-edge slope carries w0 (SG:809), so w0 moves only the center's own row."""
+    s = _small_water(7, nmol=3)
+    base = dict(function="pow", r0=1.0, c=1.0, d=1.0, m=2.0)
+    Js = {}
+    for w0 in (1.0, 0.3):
+        o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=2, sigma=0.5, species=[1, 8],
+                          weighting=dict(base, w0=w0), average="cc")
+        Js[w0] = o.derivatives(s, centers=[0, 3], return_descriptor=False)
+    diff = Js[1.0] - Js[0.3]
+    mask = np.zeros(diff.shape, bool)
+    mask[0, 0] = mask[1, 3] = True
+    assert np.abs(diff[~mask]).max() == 0.0
+    # the difference is (1 - w0) * B_n1(0) * grad Ysolid_1m in the l=1 cc slots only
+    o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=2, sigma=0.5, species=[1, 8], average="cc")
+    B, _ = o.radial_B_Bhat(np.zeros(1))
+    k = math.sqrt(3 / (4 * math.pi))
+    d = diff[0, 0].reshape(3, 2, 3, 9)           # (dir, species, n, lm) in cc order (z, x, y)
+    sp = 1                                        # atom 0 is O
+    expect = np.zeros((3, 2, 3, 9))
+    for direction, slot in ((2, 1), (0, 2), (1, 3)):
+        expect[direction, sp, :, slot] = 0.7 * B[0, :, 1] * k
+    np.testing.assert_allclose(d, expect, atol=1e-12 * np.abs(expect).max())
+
+
+def test_jacobian_periodic_fd():
+    s = systems.water((2, 1, 1), spacing=2.6, seed=8, cell_length=4.2)
+    o = so.OracleSOAP(r_cut=2.5, n_max=2, l_max=2, sigma=0.5, species=[1, 8], periodic=True)
+    _fd_check(o, s, centers=[0, 4])
+
+
+def test_jacobian_attach_fd():
+    s = _small_water(9, nmol=2)
+    o = so.OracleSOAP(r_cut=3.5, n_max=2, l_max=2, sigma=0.5, species=[1, 8])
+    J0 = _fd_check(o, s, centers=[0, 3])
+    J1 = _fd_check(o, s, centers=[0, 3], attach=True)
+    assert np.abs(J0 - J1).max() > 1e-3 * np.abs(J0).max()
+
+
+def test_planted_sign_error_is_caught(monkeypatch):
+    """Flipping the Gaussian-width term of the edge gradient must fail the FD check."""
+    s = _small_water(5, nmol=3)
+    o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=2, sigma=0.5, species=[1, 8])
+    orig = so.OracleSOAP.radial_B_Bhat
+
+    def bad(self, r):
+        B, Bh = orig(self, r)
+        return B, -Bh
+    monkeypatch.setattr(so.OracleSOAP, "radial_B_Bhat", bad)
+    with pytest.raises(AssertionError):
+        _fd_check(o, s, centers=[0, 4])
+
+
+def test_vjp_matches_dense_contraction():
+    s = _small_water(10, nmol=4)
+    for avg in ("off", "cc"):
+        o = so.OracleSOAP(r_cut=3.5, n_max=3, l_max=3, sigma=0.5, species=[1, 8], average=avg)
+        w = np.random.default_rng(1).normal(size=(s["positions"].shape[0], o.get_number_of_features()))
+        J = o.derivatives(s, return_descriptor=False)
+        ref = np.einsum("cadf,cf->ad", J, w)
+        F = o.vjp(s, w)
+        np.testing.assert_allclose(F, ref, atol=1e-11 * np.abs(ref).max())
