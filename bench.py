@@ -1,0 +1,151 @@
+"""Benchmark: SOAP create() + analytical dp/dR throughput (atom-envs/s) on one MI355X.
+
+Default workload: the non-periodic (H2O)_1000 cluster (3,000 atoms; O sites on a
+jittered 10x10x10 lattice of 3.1 A, random orientations, seed 0), species H/O,
+GTO basis, r_cut=10.0, sigma=0.5, n_max=7, l_max=3, average="off", float32
+output with the reference's float64 internal arithmetic.  One step = one
+create() of every center plus one derivatives(method="analytical") call, i.e.
+the full dense (3000, 3000, 3, 420) Jacobian, as the reference's serial driver
+does.  Positions are already resident in HBM when the timed region starts.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "atom-envs/s (create + dp/dR) at ≤1e-4 rel err: mixed/learned vs plain FP64 on GPU"
+HBM_PEAK_GBS = 8000.0
+HERE = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "jacobian_dense_h2o1000_traffic.txt")
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="float64", choices=["float64", "mixed", "float32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-centers", type=int, default=48)
+    ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the dense Jacobian kernel from the committed rocprofv3 PMC summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1].strip())
+    return None
+
+
+def cpu_baseline(system, params, n_centers):
+    """The FP64 oracle (numpy, 1 thread) on the first n_centers centers (search over all atoms)."""
+    from threadpoolctl import threadpool_limits
+    from oracle import soap_oracle as so
+    o = so.OracleSOAP(**params)
+    cen = list(range(n_centers))
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        o.create(system, centers=cen)
+        o.derivatives(system, centers=cen, return_descriptor=False)
+        dt = time.perf_counter() - t0
+    return {"value": n_centers / dt, "unit": "atom-envs/s", "cores": 1, "kind": "port",
+            "sample": f"first {n_centers} of 3000 centers of the same (H2O)_1000 workload: create + dense "
+                      f"analytical dp/dR, numpy float64 oracle, 1 thread, {dt:.1f} s"}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the work would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from cusoap_amd import SOAP, _lib, systems
+
+    dev = torch.device("cuda:0")
+    params = dict(r_cut=10.0, n_max=7, l_max=3, sigma=0.5, species=[1, 8], rbf="gto", average="off")
+    system = systems.water_cluster_3000(seed=0)
+    soap = SOAP(device=dev, dtype="float32", precision=args.precision, **params)
+    sys_dev = {"positions": torch.tensor(system["positions"], device=dev),
+               "atomic_numbers": system["atomic_numbers"]}
+    n_env = system["positions"].shape[0]
+    n_feat = soap.get_number_of_features()
+
+    def step():
+        d = soap.create(sys_dev)
+        J = soap.derivatives(sys_dev, method="analytical", return_descriptor=False)
+        return d, J
+
+    for _ in range(args.warmup):
+        d, J = step()
+        del d, J
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    for name in ("jacobian_dense", "jacobian_edges", "coefficients", "nl_count", "nl_fill", "power_spectrum"):
+        _lib.profile_get(name)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d, J = step()
+        del d, J
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _lib.profile_enable(False)
+    kern = {}
+    for name in ("jacobian_dense", "jacobian_edges", "coefficients", "nl_count", "nl_fill", "power_spectrum"):
+        ms, n = _lib.profile_get(name)
+        kern[name] = ms / max(n, 1)
+    ms_step = dt * 1e3 / args.steps
+    value = n_env * args.steps / dt
+    jac_bytes = float(n_env) * n_env * 3 * n_feat * 4          # algorithmic: every output element stored once
+    achieved = jac_bytes / (kern["jacobian_dense"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "atom-envs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "float64" else ("f64+f32" if args.precision == "mixed" else "f32"),
+        "data": "synthetic",
+        "config": {"workload": "(H2O)_1000 non-periodic cluster, 3000 atoms, H/O, GTO, r_cut=10.0, sigma=0.5, "
+                               "n_max=7, l_max=3, average=off; create + dense analytical dp/dR "
+                               "(3000x3000x3x420 float32)",
+                   "n_atoms": n_env, "n_features": n_feat, "precision": args.precision,
+                   "output_dtype": "float32", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                     "kernel": "dense_kernel (jacobian_dense)", "bytes_per_launch": jac_bytes,
+                     "avg_kernel_ms": round(kern["jacobian_dense"], 4)},
+        "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
+    }
+    if args.check:
+        from oracle import soap_oracle as so
+        ref = so.OracleSOAP(**params).create(system)
+        d = soap.create(sys_dev).double().cpu().numpy()
+        sc = np.abs(ref).max(0)
+        out["max_col_rel_err_vs_oracle"] = float((np.abs(d - ref).max(0) / np.where(sc > 0, sc, 1)).max())
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(system, params, args.cpu_centers)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+// extern "C" entry points of libcusoap_amd.so (declared in include/cusoap_amd.h).
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cusoap {
+
+static thread_local std::string g_last_error;
+
+// ---- event profiler ---------------------------------------------------------
+struct ProfSpan {
+    hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::map<std::string, std::vector<ProfSpan>> g_prof;
+static std::map<std::string, hipEvent_t> g_prof_open;
+
+void prof_begin(const char* name, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, st);
+    g_prof_open[name] = e;
+}
+
+void prof_end(const char* name, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_on) return;
+    auto it = g_prof_open.find(name);
+    if (it == g_prof_open.end()) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, st);
+    g_prof[name].push_back({it->second, e});
+    g_prof_open.erase(it);
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int nl_plan_impl(const double*, int32_t, int32_t, int32_t, double, void*, cusoap_nl_plan_t*, hipStream_t);
+int nl_count_impl(const cusoap_nl_plan_t*, const double*, int32_t, const double*, int32_t, const double*, int32_t,
+                  double, int32_t, void*, size_t, int64_t*, int64_t*, int32_t*, hipStream_t);
+int nl_fill_impl(const cusoap_nl_plan_t*, const double*, int32_t, const double*, int32_t, const double*, int32_t,
+                 double, int32_t, void*, size_t, const int64_t*, int32_t, int32_t*, hipStream_t);
+int coefficients_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int,
+                      const double*, int, const int32_t*, const int64_t*, const int32_t*, int, void*, hipStream_t);
+int power_spectrum_impl(const cusoap_basis_t*, const void*, int, int, const uint32_t*, int, void*, int, hipStream_t);
+int cc_projection_impl(const cusoap_basis_t*, const void*, int, int, void*, int, hipStream_t);
+size_t jacobian_ws_bytes(const cusoap_basis_t*, int, int64_t);
+int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int,
+                        const double*, int, const int32_t*, const int64_t*, const int32_t*, int64_t, const double*,
+                        const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, hipStream_t);
+size_t vjp_ws_bytes(const cusoap_basis_t*, int);
+int vjp_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int, const double*, int,
+             const int32_t*, const int64_t*, const int32_t*, const double*, const int32_t*, const void*, int, int, int,
+             void*, size_t, double*, hipStream_t);
+int lowdin_impl(const double*, int, int, double, double*, hipStream_t);
+
+}  // namespace cusoap
+
+using namespace cusoap;
+
+extern "C" {
+
+const char* cusoap_last_error(void) { return g_last_error.c_str(); }
+int cusoap_version(void) { return 1; }
+int cusoap_max_l(void) { return kMaxL; }
+int cusoap_max_n(void) { return kMaxN; }
+
+void cusoap_profile_enable(int32_t on) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = on != 0;
+}
+
+int cusoap_profile_get(const char* name, double* total_ms, int32_t* count) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    double tot = 0.0;
+    int n = 0;
+    auto it = g_prof.find(name);
+    if (it != g_prof.end()) {
+        for (auto& sp : it->second) {
+            float ms = 0.f;
+            if (hipEventSynchronize(sp.b) != hipSuccess) return fail(CUSOAP_ERR_HIP, "profile: event sync failed");
+            if (hipEventElapsedTime(&ms, sp.a, sp.b) == hipSuccess) {
+                tot += ms;
+                ++n;
+            }
+            hipEventDestroy(sp.a);
+            hipEventDestroy(sp.b);
+        }
+        g_prof.erase(it);
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = n;
+    return CUSOAP_OK;
+}
+
+int cusoap_nl_plan(const double* centers, int32_t n_centers, int32_t n_atoms, int32_t n_img, double cutoff, void* tmp,
+                   cusoap_nl_plan_t* plan, void* stream) {
+    return nl_plan_impl(centers, n_centers, n_atoms, n_img, cutoff, tmp, plan, (hipStream_t)stream);
+}
+
+int cusoap_nl_count(const cusoap_nl_plan_t* plan, const double* positions, int32_t n_atoms, const double* shifts,
+                    int32_t n_img, const double* centers, int32_t n_centers, double cutoff, int32_t drop_self,
+                    void* ws, size_t ws_bytes, int64_t* offsets, int64_t* total_host, int32_t* max_count_host,
+                    void* stream) {
+    return nl_count_impl(plan, positions, n_atoms, shifts, n_img, centers, n_centers, cutoff, drop_self, ws, ws_bytes,
+                         offsets, total_host, max_count_host, (hipStream_t)stream);
+}
+
+int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_t n_atoms, const double* shifts,
+                   int32_t n_img, const double* centers, int32_t n_centers, double cutoff, int32_t drop_self, void* ws,
+                   size_t ws_bytes, const int64_t* offsets, int32_t max_count, int32_t* nbr, void* stream) {
+    return nl_fill_impl(plan, positions, n_atoms, shifts, n_img, centers, n_centers, cutoff, drop_self, ws, ws_bytes,
+                        offsets, max_count, nbr, (hipStream_t)stream);
+}
+
+int cusoap_coefficients(const cusoap_basis_t* basis, int32_t precision, const double* positions, int32_t n_atoms,
+                        const int32_t* atom_species, const double* shifts, int32_t n_img, const double* centers,
+                        int32_t n_centers, const int32_t* center_self_species, const int64_t* offsets,
+                        const int32_t* nbr, int32_t create_path, void* coeffs, void* stream) {
+    return coefficients_impl(basis, precision, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
+                             center_self_species, offsets, nbr, create_path, coeffs, (hipStream_t)stream);
+}
+
+int cusoap_power_spectrum(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype, int32_t n_centers,
+                          const uint32_t* feat_table, int32_t n_feat, void* out, int32_t out_dtype, void* stream) {
+    return power_spectrum_impl(basis, coeffs, coeffs_dtype, n_centers, feat_table, n_feat, out, out_dtype,
+                               (hipStream_t)stream);
+}
+
+int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype, int32_t n_centers,
+                         void* out, int32_t out_dtype, void* stream) {
+    return cc_projection_impl(basis, coeffs, coeffs_dtype, n_centers, out, out_dtype, (hipStream_t)stream);
+}
+
+int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges, size_t* bytes) {
+    if (!basis || !bytes) return fail(CUSOAP_ERR_ARG, "jacobian_workspace_size: null argument");
+    *bytes = jacobian_ws_bytes(basis, mode, n_edges);
+    return CUSOAP_OK;
+}
+
+int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode, const double* positions, int32_t n_atoms,
+                          const int32_t* atom_species, const double* shifts, int32_t n_img, const double* centers,
+                          int32_t n_centers, const int32_t* center_attach, const int64_t* offsets, const int32_t* nbr,
+                          int64_t n_edges, const double* coeffs, const int32_t* inv_atom, int32_t n_inc,
+                          const uint32_t* feat_table, int32_t n_feat, void* ws, size_t ws_bytes, void* out,
+                          int32_t out_dtype, void* stream) {
+    return jacobian_dense_impl(basis, mode, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
+                               center_attach, offsets, nbr, n_edges, coeffs, inv_atom, n_inc, feat_table, n_feat, ws,
+                               ws_bytes, out, out_dtype, (hipStream_t)stream);
+}
+
+int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes) {
+    if (!basis || !bytes) return fail(CUSOAP_ERR_ARG, "vjp_workspace_size: null argument");
+    *bytes = vjp_ws_bytes(basis, n_centers);
+    return CUSOAP_OK;
+}
+
+int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, const double* positions, int32_t n_atoms,
+               const int32_t* atom_species, const double* shifts, int32_t n_img, const double* centers,
+               int32_t n_centers, const int32_t* center_attach, const int64_t* offsets, const int32_t* nbr,
+               const double* coeffs, const int32_t* pair_feature, const void* grad_desc, int32_t gd_dtype,
+               int32_t gd_broadcast, int32_t n_feat, void* ws, size_t ws_bytes, double* grad_pos, void* stream) {
+    return vjp_impl(basis, mode, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers, center_attach,
+                    offsets, nbr, coeffs, pair_feature, grad_desc, gd_dtype, gd_broadcast, n_feat, ws, ws_bytes,
+                    grad_pos, (hipStream_t)stream);
+}
+
+int cusoap_lowdin_invsqrt_f64(const double* S, int32_t batch, int32_t n, double eps, double* out, void* stream) {
+    return lowdin_impl(S, batch, n, eps, out, (hipStream_t)stream);
+}
+
+}  // extern "C"

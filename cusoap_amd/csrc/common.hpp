@@ -1,0 +1,202 @@
+// Shared device code of the cusoap_amd kernels: error plumbing, real solid
+// harmonics, radial functions and the weighting function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/cusoap_amd.h"
+#include "sh_constants.inc"
+
+namespace cusoap {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define CUSOAP_HIP(expr)                                                              \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess)                                                         \
+            return ::cusoap::fail(CUSOAP_ERR_HIP, std::string(#expr) + ": " +         \
+                                                      hipGetErrorString(_e));         \
+    } while (0)
+
+#define CUSOAP_LAUNCH_CHECK()                                                         \
+    do {                                                                              \
+        hipError_t _e = hipGetLastError();                                            \
+        if (_e != hipSuccess)                                                         \
+            return ::cusoap::fail(CUSOAP_ERR_HIP, std::string("kernel launch: ") +    \
+                                                      hipGetErrorString(_e));         \
+    } while (0)
+
+// Optional per-kernel timing with HIP events recorded on the launch stream.
+void prof_begin(const char* name, hipStream_t st);
+void prof_end(const char* name, hipStream_t st);
+
+constexpr int kMaxL = 9;        // compiled l_max range 0..kMaxL
+constexpr int kMaxN = 16;       // n_max range 1..kMaxN
+constexpr int kMaxS = 64;       // species
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// real solid harmonics r^l Y_lm(r) and Cartesian gradients
+// ---------------------------------------------------------------------------
+// Associated-Legendre recurrence in (z, r^2) for Q_l^m, with (x + i y)^m for the
+// azimuthal part:  Y_l0 = K_l0 Q_l^0,  Y_l,+m = K_lm Q_l^m Re(x+iy)^m,
+// Y_l,-m = K_lm Q_l^m Im(x+iy)^m.  Gradients by forward differentiation of the
+// recurrence (dQ/dz at fixed r^2 and dQ/dr^2).  `put(lm, v, gx, gy, gz)` is
+// called once per lm.
+template <int LMAX, bool GRAD, typename T, typename Put>
+__device__ __forceinline__ void solid_harmonics(T x, T y, T z, Put&& put) {
+    const T r2 = (x * x + y * y) + z * z;
+    T cm = T(1), sm = T(0);          // (x+iy)^m
+    T cm1 = T(0), sm1 = T(0);        // (x+iy)^(m-1)
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) {
+        if (m > 0) {
+            const T nc = x * cm - y * sm;
+            const T ns = x * sm + y * cm;
+            cm1 = cm;
+            sm1 = sm;
+            cm = nc;
+            sm = ns;
+        }
+        T q2 = T(0), q2z = T(0), q2r = T(0);        // Q_{l-2}
+        T q1 = T(kDoubleFact[m]), q1z = T(0), q1r = T(0);   // Q_{l-1} (starts at Q_m^m)
+#pragma unroll
+        for (int l = m; l <= LMAX; ++l) {
+            T q, qz, qr;
+            if (l == m) {
+                q = q1;
+                qz = T(0);
+                qr = T(0);
+            } else if (l == m + 1) {
+                q = T(2 * m + 1) * z * q1;
+                qz = T(2 * m + 1) * q1;
+                qr = T(0);
+            } else {
+                const T a = T(2 * l - 1), b = T(l + m - 1), inv = T(1.0 / (l - m));
+                q = (a * z * q1 - b * r2 * q2) * inv;
+                qz = (a * (q1 + z * q1z) - b * r2 * q2z) * inv;
+                qr = (a * z * q1r - b * (q2 + r2 * q2r)) * inv;
+            }
+            if (l > m) {
+                q2 = q1; q2z = q1z; q2r = q1r;
+                q1 = q;  q1z = qz;  q1r = qr;
+            }
+            const T K = T(kSHNorm[l][m]);
+            const int c0 = l * l + l;
+            if (m == 0) {
+                T gx = T(0), gy = T(0), gz = T(0);
+                if (GRAD) {
+                    gx = K * T(2) * x * qr;
+                    gy = K * T(2) * y * qr;
+                    gz = K * (qz + T(2) * z * qr);
+                }
+                put(c0, K * q, gx, gy, gz);
+            } else {
+                T gxp = T(0), gyp = T(0), gzp = T(0), gxm = T(0), gym = T(0), gzm = T(0);
+                if (GRAD) {
+                    const T dqx = T(2) * x * qr, dqy = T(2) * y * qr, dqz = qz + T(2) * z * qr;
+                    const T fm = T(m);
+                    gxp = K * (dqx * cm + q * fm * cm1);
+                    gyp = K * (dqy * cm - q * fm * sm1);
+                    gzp = K * dqz * cm;
+                    gxm = K * (dqx * sm + q * fm * sm1);
+                    gym = K * (dqy * sm + q * fm * cm1);
+                    gzm = K * dqz * sm;
+                }
+                put(c0 + m, K * q * cm, gxp, gyp, gzp);
+                put(c0 - m, K * q * sm, gxm, gym, gzm);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weighting (w(r) and dw/dr; w0 and zero slope below 1e-12)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void weighting(const cusoap_basis_t& b, double r, double& w, double& dw) {
+    if (r < 1e-12) {
+        w = b.w0;
+        dw = 0.0;
+        return;
+    }
+    switch (b.weighting) {
+        case CUSOAP_W_POLY: {
+            const double x = r / b.w_r0;
+            const double base = 1.0 + 2.0 * (x * x * x) - 3.0 * (x * x);
+            if (r <= b.w_r0 && base > 0.0) {
+                w = b.w_c * pow(base, b.w_m);
+                dw = b.w_c * b.w_m * pow(base, b.w_m - 1.0) * (6.0 * x * x - 6.0 * x) / b.w_r0;
+            } else {
+                w = 0.0;
+                dw = 0.0;
+            }
+            return;
+        }
+        case CUSOAP_W_POW: {
+            const double x = r / b.w_r0;
+            const double xm = pow(x, b.w_m);
+            const double den = b.w_d + xm;
+            w = b.w_c / den;
+            dw = -b.w_c * (b.w_m * pow(x, b.w_m - 1.0) / b.w_r0) / (den * den);
+            return;
+        }
+        case CUSOAP_W_EXP: {
+            const double e = exp(-r / b.w_r0);
+            const double den = b.w_d + e;
+            w = b.w_c / den;
+            dw = b.w_c * e / b.w_r0 / (den * den);
+            return;
+        }
+        default:
+            w = 1.0;
+            dw = 0.0;
+            return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// polynomial basis: I_nl(r), J_nl(r) by the clamped Bessel recurrence on the
+// Gauss-Legendre grid (restates _poly_I_J, soap_generator.py:1437-1507)
+// ---------------------------------------------------------------------------
+// Calls acc(q, l, P) for l = 0..ltop with P = common * ie_l(t_q).
+template <typename Acc>
+__device__ __forceinline__ void poly_levels(const cusoap_basis_t& b, double r, int ltop, Acc&& acc) {
+    const double eta = b.eta;
+    for (int q = 0; q < b.quad_n; ++q) {
+        const double rx = b.poly_rx[q];
+        double tt = (2.0 * eta) * rx * r;
+        tt = tt < 1e-12 ? 1e-12 : tt;
+        const double e2 = exp(-2.0 * tt);
+        const double oOt = 1.0 / tt;
+        const double d = rx - r;
+        const double common = b.poly_rxw[q] * exp(-eta * (d * d));
+        double Ppp = 0.0, Pp = 0.0;
+        for (int l = 0; l <= ltop; ++l) {
+            double P;
+            if (l == 0) {
+                P = common * ((1.0 - e2) * (0.5 * oOt));
+            } else if (l == 1) {
+                double ie1 = (1.0 + e2) * (0.5 * oOt) - (1.0 - e2) * (0.5 * oOt * oOt);
+                ie1 = ie1 < 0.0 ? 0.0 : ie1;
+                P = common * ie1;
+            } else {
+                double v = Ppp - (double)(2 * l - 1) * oOt * Pp;
+                v = v < 0.0 ? 0.0 : v;
+                P = v < Pp ? v : Pp;
+            }
+            acc(q, l, P);
+            Ppp = Pp;
+            Pp = P;
+        }
+    }
+}
+
+}  // namespace cusoap

@@ -1,0 +1,490 @@
+// Cell-list neighbour search with periodic images (replaces the reference's
+// chunked GEMM radius search, soap_generator.py:1042-1181).
+//
+// Pipeline (all on the caller's stream):
+//   plan   : bounding box of the centers -> uniform grid of cells >= cutoff
+//   count  : bin every image atom (ext = img * N + atom) that falls in the grid
+//            (counting sort), then one wavefront per center scans the 27
+//            surrounding cells and counts edges with a ballot
+//   scan   : exclusive scan of the counts -> CSR offsets
+//   fill   : one workgroup per center gathers its edges into LDS and
+//            bitonic-sorts them by ext index, which is the order of the
+//            reference's `nonzero` over (center, ext)
+// The distance test restates the reference formula
+//   d^2 = ((-2 c.n) + |c|^2) + |n|^2 <= cutoff^2
+// in float64 with explicit round-to-nearest operations (no contraction), so
+// the edge set is the one that formula defines.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace cusoap {
+
+namespace {
+
+__device__ __forceinline__ double sqnorm_rn(double x, double y, double z) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+__device__ __forceinline__ double dist2_formula(double cx, double cy, double cz, double cn,
+                                                double nx, double ny, double nz, double nn) {
+    const double dot = __dadd_rn(__dadd_rn(__dmul_rn(cx, nx), __dmul_rn(cy, ny)), __dmul_rn(cz, nz));
+    return __dadd_rn(__dadd_rn(__dmul_rn(dot, -2.0), cn), nn);
+}
+
+struct Grid {
+    double ox, oy, oz, hx, hy, hz;
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ Grid make_grid(const cusoap_nl_plan_t& p) {
+    Grid g;
+    g.ox = p.origin[0]; g.oy = p.origin[1]; g.oz = p.origin[2];
+    g.hx = p.cell[0];   g.hy = p.cell[1];   g.hz = p.cell[2];
+    g.nx = p.dims[0];   g.ny = p.dims[1];   g.nz = p.dims[2];
+    return g;
+}
+
+__device__ __forceinline__ int cell_coord(double v, double o, double h, int n) {
+    const double f = floor((v - o) / h);
+    if (!(f >= 0.0) || f >= (double)n) return -1;
+    return (int)f;
+}
+
+// --- workspace layout -------------------------------------------------------
+struct NLWs {
+    int32_t* ext_cell;     // (n_ext)
+    int32_t* cell_count;   // (n_cells)
+    int64_t* cell_start;   // (n_cells + 1)
+    double4* binned;       // (n_ext)  x, y, z, |p|^2
+    int32_t* binned_ext;   // (n_ext)
+    int32_t* counts;       // (n_centers)
+    int64_t* partials;     // scan partials
+    int32_t* max_count;    // (1)
+    size_t bytes;
+};
+
+constexpr int kScanBlock = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+size_t scan_partials(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+NLWs carve(void* base, int64_t n_ext, int32_t n_cells, int32_t n_centers) {
+    NLWs w;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align_up(off + bytes, 256);
+        return (char*)base + o;
+    };
+    w.ext_cell = (int32_t*)take(sizeof(int32_t) * n_ext);
+    w.cell_count = (int32_t*)take(sizeof(int32_t) * (n_cells + 1));
+    w.cell_start = (int64_t*)take(sizeof(int64_t) * (n_cells + 1));
+    w.binned = (double4*)take(sizeof(double4) * n_ext);
+    w.binned_ext = (int32_t*)take(sizeof(int32_t) * n_ext);
+    w.counts = (int32_t*)take(sizeof(int32_t) * (n_centers + 1));
+    int64_t mx = std::max<int64_t>(n_cells + 1, n_centers + 1);
+    w.partials = (int64_t*)take(sizeof(int64_t) * (scan_partials(mx) + kScanTile));
+    w.max_count = (int32_t*)take(sizeof(int32_t) * 4);
+    w.bytes = off;
+    return w;
+}
+
+// --- bounding box -------------------------------------------------------------
+__global__ void bbox_partial_kernel(const double* __restrict__ c, int n, double* __restrict__ part) {
+    __shared__ double red[6][256];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int d = 0; d < 3; ++d) {
+            const double v = c[3 * (size_t)i + d];
+            mn[d] = fmin(mn[d], v);
+            mx[d] = fmax(mx[d], v);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = mn[d];
+        red[3 + d][threadIdx.x] = mx[d];
+    }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int d = 0; d < 3; ++d) {
+                red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+                red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 6; ++d) part[6 * blockIdx.x + d] = red[d][0];
+}
+
+__global__ void bbox_final_kernel(double* part, int nblk) {
+    if (threadIdx.x != 0) return;
+    double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nblk; ++b)
+        for (int d = 0; d < 3; ++d) {
+            r[d] = fmin(r[d], part[6 * b + d]);
+            r[3 + d] = fmax(r[3 + d], part[6 * b + 3 + d]);
+        }
+    for (int d = 0; d < 6; ++d) part[d] = r[d];
+}
+
+// --- exclusive scan int32 -> int64 ---------------------------------------------
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t t = __shfl_up(v, o, 64);
+        if ((int)(threadIdx.x & 63) >= o) v += t;
+    }
+    return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns exclusive prefix, sets total
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* wsum, int64_t& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int64_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        int64_t s = lane < nw ? wsum[lane] : 0;
+        s = wave_incl_scan(s);
+        if (lane < nw) wsum[lane] = s;
+    }
+    __syncthreads();
+    const int64_t base = wid > 0 ? wsum[wid - 1] : 0;
+    total = wsum[nw - 1];
+    __syncthreads();
+    return base + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_tiles_kernel(const int32_t* __restrict__ in, int64_t n,
+                                                                int64_t* __restrict__ out,
+                                                                int64_t* __restrict__ partials) {
+    __shared__ int64_t wsum[kScanBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t v[kScanItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? (int64_t)in[base + k] : 0;
+        s += v[k];
+    }
+    int64_t total;
+    int64_t pre = block_excl_scan(s, wsum, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_partials_kernel(int64_t* partials, int64_t nb) {
+    __shared__ int64_t wsum[kScanBlock / 64];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += kScanBlock) {
+        const int64_t i = b0 + threadIdx.x;
+        const int64_t v = i < nb ? partials[i] : 0;
+        int64_t total;
+        const int64_t pre = block_excl_scan(v, wsum, total);
+        if (i < nb) partials[i] = carry + pre;
+        carry += total;
+    }
+    if (threadIdx.x == 0) partials[nb] = carry;
+}
+
+__global__ void scan_add_kernel(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ partials,
+                                int64_t nb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += partials[i / kScanTile];
+    if (i == 0) out[n] = partials[nb];
+}
+
+int scan_exclusive(const int32_t* in, int64_t n, int64_t* out, int64_t* partials, hipStream_t st) {
+    const int64_t nb = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
+    scan_tiles_kernel<<<(unsigned)nb, kScanBlock, 0, st>>>(in, n, out, partials);
+    scan_partials_kernel<<<1, kScanBlock, 0, st>>>(partials, nb);
+    const int64_t threads = std::max<int64_t>(n, 1);
+    scan_add_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(out, n, partials, nb);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
+// --- binning --------------------------------------------------------------------
+__global__ void bin_kernel(cusoap_nl_plan_t plan, const double* __restrict__ pos, int n_atoms,
+                           const double* __restrict__ shifts, int64_t n_ext,
+                           int32_t* __restrict__ ext_cell, int32_t* __restrict__ cell_count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_ext) return;
+    const Grid g = make_grid(plan);
+    const int img = (int)(i / n_atoms), a = (int)(i - (int64_t)img * n_atoms);
+    const double px = pos[3 * (size_t)a + 0] + shifts[3 * img + 0];
+    const double py = pos[3 * (size_t)a + 1] + shifts[3 * img + 1];
+    const double pz = pos[3 * (size_t)a + 2] + shifts[3 * img + 2];
+    const int ix = cell_coord(px, g.ox, g.hx, g.nx);
+    const int iy = cell_coord(py, g.oy, g.hy, g.ny);
+    const int iz = cell_coord(pz, g.oz, g.hz, g.nz);
+    int cell = -1;
+    if (ix >= 0 && iy >= 0 && iz >= 0) {
+        cell = (ix * g.ny + iy) * g.nz + iz;
+        atomicAdd(&cell_count[cell], 1);
+    }
+    ext_cell[i] = cell;
+}
+
+__global__ void scatter_kernel(const double* __restrict__ pos, int n_atoms, const double* __restrict__ shifts,
+                               int64_t n_ext, const int32_t* __restrict__ ext_cell,
+                               const int64_t* __restrict__ cell_start, int32_t* __restrict__ cursor,
+                               double4* __restrict__ binned, int32_t* __restrict__ binned_ext) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_ext) return;
+    const int cell = ext_cell[i];
+    if (cell < 0) return;
+    const int img = (int)(i / n_atoms), a = (int)(i - (int64_t)img * n_atoms);
+    const double px = pos[3 * (size_t)a + 0] + shifts[3 * img + 0];
+    const double py = pos[3 * (size_t)a + 1] + shifts[3 * img + 1];
+    const double pz = pos[3 * (size_t)a + 2] + shifts[3 * img + 2];
+    const int64_t slot = cell_start[cell] + atomicAdd(&cursor[cell], 1);
+    binned[slot] = make_double4(px, py, pz, sqnorm_rn(px, py, pz));
+    binned_ext[slot] = (int32_t)i;
+}
+
+struct CenterCells {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+__device__ __forceinline__ CenterCells center_cells(const Grid& g, double cx, double cy, double cz) {
+    auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    int ix = clampi((int)floor((cx - g.ox) / g.hx), 0, g.nx - 1);
+    int iy = clampi((int)floor((cy - g.oy) / g.hy), 0, g.ny - 1);
+    int iz = clampi((int)floor((cz - g.oz) / g.hz), 0, g.nz - 1);
+    CenterCells r;
+    r.x0 = ix > 0 ? ix - 1 : 0; r.x1 = ix < g.nx - 1 ? ix + 1 : g.nx - 1;
+    r.y0 = iy > 0 ? iy - 1 : 0; r.y1 = iy < g.ny - 1 ? iy + 1 : g.ny - 1;
+    r.z0 = iz > 0 ? iz - 1 : 0; r.z1 = iz < g.nz - 1 ? iz + 1 : g.nz - 1;
+    return r;
+}
+
+__device__ __forceinline__ bool edge_test(double cx, double cy, double cz, double cn, const double4& p,
+                                          double cutoff2, int drop_self) {
+    const double d2 = dist2_formula(cx, cy, cz, cn, p.x, p.y, p.z, p.w);
+    if (!(d2 <= cutoff2)) return false;
+    if (drop_self) {
+        const double rx = p.x - cx, ry = p.y - cy, rz = p.z - cz;
+        if (!(sqrt(sqnorm_rn(rx, ry, rz)) > 1e-12)) return false;
+    }
+    return true;
+}
+
+// one wavefront per center
+__global__ __launch_bounds__(256) void count_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
+                                                    int n_centers, double cutoff2, int drop_self,
+                                                    const int64_t* __restrict__ cell_start,
+                                                    const double4* __restrict__ binned,
+                                                    int32_t* __restrict__ counts, int32_t* __restrict__ max_count) {
+    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= n_centers) return;
+    const Grid g = make_grid(plan);
+    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
+    const double cn = sqnorm_rn(cx, cy, cz);
+    const CenterCells cc = center_cells(g, cx, cy, cz);
+    int cnt = 0;
+    for (int ix = cc.x0; ix <= cc.x1; ++ix)
+        for (int iy = cc.y0; iy <= cc.y1; ++iy)
+            for (int iz = cc.z0; iz <= cc.z1; ++iz) {
+                const int cell = (ix * g.ny + iy) * g.nz + iz;
+                const int64_t s = cell_start[cell], e = cell_start[cell + 1];
+                for (int64_t k = s + lane; k < e; k += 64)
+                    cnt += edge_test(cx, cy, cz, cn, binned[k], cutoff2, drop_self) ? 1 : 0;
+            }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (lane == 0) {
+        counts[c] = cnt;
+        atomicMax(max_count, cnt);
+    }
+}
+
+template <int CAP>
+__global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
+                                                   int n_centers, double cutoff2, int drop_self,
+                                                   const int64_t* __restrict__ cell_start,
+                                                   const double4* __restrict__ binned,
+                                                   const int32_t* __restrict__ binned_ext,
+                                                   const int64_t* __restrict__ offsets,
+                                                   int32_t* __restrict__ nbr) {
+    __shared__ int32_t keys[CAP];
+    __shared__ int nkeys;
+    const int c = blockIdx.x;
+    if (c >= n_centers) return;
+    if (threadIdx.x == 0) nkeys = 0;
+    __syncthreads();
+    const Grid g = make_grid(plan);
+    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
+    const double cn = sqnorm_rn(cx, cy, cz);
+    const CenterCells cc = center_cells(g, cx, cy, cz);
+    for (int ix = cc.x0; ix <= cc.x1; ++ix)
+        for (int iy = cc.y0; iy <= cc.y1; ++iy)
+            for (int iz = cc.z0; iz <= cc.z1; ++iz) {
+                const int cell = (ix * g.ny + iy) * g.nz + iz;
+                const int64_t s = cell_start[cell], e = cell_start[cell + 1];
+                for (int64_t k = s + threadIdx.x; k < e; k += blockDim.x) {
+                    if (edge_test(cx, cy, cz, cn, binned[k], cutoff2, drop_self)) {
+                        const int slot = atomicAdd(&nkeys, 1);
+                        if (slot < CAP) keys[slot] = binned_ext[k];
+                    }
+                }
+            }
+    __syncthreads();
+    const int n = nkeys < CAP ? nkeys : CAP;
+    int p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    for (int i = n + threadIdx.x; i < p2; i += blockDim.x) keys[i] = INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= p2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < p2; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int a = keys[i], b = keys[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        keys[i] = b;
+                        keys[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t base = offsets[c];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) nbr[base + i] = keys[i];
+}
+
+}  // namespace
+
+int nl_plan_impl(const double* centers, int32_t n_centers, int32_t n_atoms, int32_t n_img, double cutoff,
+                 void* tmp, cusoap_nl_plan_t* plan, hipStream_t st) {
+    if (!plan || !tmp || n_centers < 0 || n_atoms < 0 || n_img < 1 || !(cutoff > 0))
+        return fail(CUSOAP_ERR_ARG, "nl_plan: invalid argument");
+    const int64_t n_ext = (int64_t)n_img * n_atoms;
+    if (n_ext >= INT_MAX) return fail(CUSOAP_ERR_UNSUPPORTED, "nl_plan: n_images * n_atoms must be < 2^31");
+    double bb[6] = {0, 0, 0, 0, 0, 0};
+    if (n_centers > 0) {
+        const int nblk = std::min(256, std::max(1, (n_centers + 255) / 256));
+        bbox_partial_kernel<<<nblk, 256, 0, st>>>(centers, n_centers, (double*)tmp);
+        bbox_final_kernel<<<1, 64, 0, st>>>((double*)tmp, nblk);
+        CUSOAP_LAUNCH_CHECK();
+        CUSOAP_HIP(hipMemcpyAsync(bb, tmp, sizeof(bb), hipMemcpyDeviceToHost, st));
+        CUSOAP_HIP(hipStreamSynchronize(st));
+    }
+    const double margin = cutoff * (1.0 + 1e-9) + 1e-6;
+    int64_t dims[3];
+    double lo[3], ext[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = bb[d] - margin;
+        ext[d] = (bb[3 + d] - bb[d]) + 2.0 * margin;
+        if (!std::isfinite(lo[d]) || !std::isfinite(ext[d]))
+            return fail(CUSOAP_ERR_ARG, "nl_plan: non-finite center coordinates");
+        dims[d] = std::max<int64_t>(1, std::min<int64_t>((int64_t)std::floor(ext[d] / margin), 1 << 12));
+    }
+    const int64_t cap = std::max<int64_t>(64, std::min<int64_t>(1 << 24, 2 * std::max<int64_t>(n_ext, 1)));
+    while (dims[0] * dims[1] * dims[2] > cap) {
+        int d = 0;
+        if (dims[1] > dims[d]) d = 1;
+        if (dims[2] > dims[d]) d = 2;
+        dims[d] = (dims[d] + 1) / 2;
+    }
+    for (int d = 0; d < 3; ++d) {
+        plan->origin[d] = lo[d];
+        plan->dims[d] = (int32_t)dims[d];
+        plan->cell[d] = ext[d] / (double)dims[d];
+    }
+    plan->n_cells = (int32_t)(dims[0] * dims[1] * dims[2]);
+    plan->n_ext = n_ext;
+    plan->ws_bytes = carve(nullptr, n_ext, plan->n_cells, n_centers).bytes;
+    return CUSOAP_OK;
+}
+
+int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_atoms, const double* shifts,
+                  int32_t n_img, const double* centers, int32_t n_centers, double cutoff, int32_t drop_self,
+                  void* ws, size_t ws_bytes, int64_t* offsets, int64_t* total_host, int32_t* max_host,
+                  hipStream_t st) {
+    if (!plan || (int64_t)n_img * n_atoms != plan->n_ext)
+        return fail(CUSOAP_ERR_ARG, "nl_count: plan does not match n_atoms * n_img");
+    NLWs w = carve(ws, plan->n_ext, plan->n_cells, n_centers);
+    if (ws_bytes < w.bytes) return fail(CUSOAP_ERR_WORKSPACE, "nl_count: workspace too small");
+    CUSOAP_HIP(hipMemsetAsync(w.cell_count, 0, sizeof(int32_t) * (plan->n_cells + 1), st));
+    CUSOAP_HIP(hipMemsetAsync(w.max_count, 0, sizeof(int32_t) * 4, st));
+    const int64_t n_ext = plan->n_ext;
+    if (n_ext > 0) {
+        bin_kernel<<<(unsigned)((n_ext + 255) / 256), 256, 0, st>>>(*plan, pos, n_atoms, shifts, n_ext, w.ext_cell,
+                                                                    w.cell_count);
+        CUSOAP_LAUNCH_CHECK();
+    }
+    int rc = scan_exclusive(w.cell_count, plan->n_cells, w.cell_start, w.partials, st);
+    if (rc) return rc;
+    CUSOAP_HIP(hipMemsetAsync(w.cell_count, 0, sizeof(int32_t) * (plan->n_cells + 1), st));
+    if (n_ext > 0) {
+        scatter_kernel<<<(unsigned)((n_ext + 255) / 256), 256, 0, st>>>(pos, n_atoms, shifts, n_ext, w.ext_cell,
+                                                                        w.cell_start, w.cell_count, w.binned,
+                                                                        w.binned_ext);
+        CUSOAP_LAUNCH_CHECK();
+    }
+    const double cutoff2 = cutoff * cutoff;
+    if (n_centers > 0) {
+        prof_begin("nl_count", st);
+        count_kernel<<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
+                                                          w.cell_start, w.binned, w.counts, w.max_count);
+        prof_end("nl_count", st);
+        CUSOAP_LAUNCH_CHECK();
+    }
+    rc = scan_exclusive(w.counts, n_centers, offsets, w.partials, st);
+    if (rc) return rc;
+    int64_t total = 0;
+    int32_t mx = 0;
+    CUSOAP_HIP(hipMemcpyAsync(&total, offsets + n_centers, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    CUSOAP_HIP(hipMemcpyAsync(&mx, w.max_count, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CUSOAP_HIP(hipStreamSynchronize(st));
+    if (total_host) *total_host = total;
+    if (max_host) *max_host = mx;
+    return CUSOAP_OK;
+}
+
+int nl_fill_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_atoms, const double* shifts,
+                 int32_t n_img, const double* centers, int32_t n_centers, double cutoff, int32_t drop_self,
+                 void* ws, size_t ws_bytes, const int64_t* offsets, int32_t max_count, int32_t* nbr,
+                 hipStream_t st) {
+    (void)pos; (void)shifts; (void)n_img; (void)n_atoms;
+    NLWs w = carve(ws, plan->n_ext, plan->n_cells, n_centers);
+    if (ws_bytes < w.bytes) return fail(CUSOAP_ERR_WORKSPACE, "nl_fill: workspace too small");
+    if (n_centers == 0 || max_count == 0) return CUSOAP_OK;
+    const double cutoff2 = cutoff * cutoff;
+    if (max_count <= 1024) {
+        prof_begin("nl_fill", st);
+        fill_kernel<1024><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self, w.cell_start,
+                                                     w.binned, w.binned_ext, offsets, nbr);
+        prof_end("nl_fill", st);
+    } else if (max_count <= 4096) {
+        prof_begin("nl_fill", st);
+        fill_kernel<4096><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self, w.cell_start,
+                                                     w.binned, w.binned_ext, offsets, nbr);
+        prof_end("nl_fill", st);
+    } else if (max_count <= 16384) {
+        prof_begin("nl_fill", st);
+        fill_kernel<16384><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
+                                                      w.cell_start, w.binned, w.binned_ext, offsets, nbr);
+        prof_end("nl_fill", st);
+    } else {
+        return fail(CUSOAP_ERR_CAPACITY, "nl_fill: a center has more than 16384 neighbours within the cutoff");
+    }
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
+}  // namespace cusoap

@@ -1,0 +1,847 @@
+"""MI355X-native drop-in for the reference ``SOAP`` class (src/cusoap/soap_generator.py).
+
+Host-side parameter resolution, basis tables and input adaptation mirror the
+reference (soap_generator.py:265-1013); every per-atom stage -- neighbour
+search, density coefficients, power spectrum / cc, dense Jacobian and the
+adjoint -- runs in the hand-written HIP kernels of ``libcusoap_amd.so``
+through its C ABI.  There is no CPU fallback: ``create`` and ``derivatives``
+raise when no HIP device or no built library is available.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+from typing import Any, Dict, List, Optional, Sequence, Tuple, Union
+
+import ctypes as C
+import numpy as np
+import torch
+
+from . import _lib
+from .systems import SYMBOLS
+
+PI = math.pi
+Y00 = 1.0 / math.sqrt(4.0 * math.pi)
+
+
+# ----------------------------------------------------------------------------
+# host helpers (restate soap_generator.py:265-470)
+# ----------------------------------------------------------------------------
+
+def _torch_dtype_from_str(dtype: str) -> torch.dtype:
+    """soap_generator.py:265-270."""
+    if dtype == "float32":
+        return torch.float32
+    if dtype == "float64":
+        return torch.float64
+    raise ValueError("dtype must be 'float32' or 'float64'")
+
+
+def _host(x, dtype=None) -> np.ndarray:
+    if isinstance(x, torch.Tensor):
+        x = x.detach().cpu().numpy()
+    return np.asarray(x, dtype=dtype)
+
+
+def system_to_arrays(system: Any, keep_device: bool = False):
+    """ASE-like object, dict or (positions, Z[, cell[, pbc]]) -> arrays (soap_generator.py:291-332).
+
+    Positions come back as float64 host arrays, or -- with keep_device and a
+    GPU tensor as input -- as the float64 device tensor itself (no host copy).
+    """
+    if isinstance(system, (tuple, list)):
+        if len(system) < 2:
+            raise ValueError("Tuple/list system must be (positions, atomic_numbers, [cell], [pbc]).")
+        pos, Z = system[0], system[1]
+        cell = system[2] if len(system) >= 3 else None
+        pbc = system[3] if len(system) >= 4 else None
+    elif isinstance(system, dict):
+        pos, Z = system["positions"], system["atomic_numbers"]
+        cell, pbc = system.get("cell"), system.get("pbc")
+    elif hasattr(system, "get_positions") and hasattr(system, "get_atomic_numbers"):
+        pos, Z = system.get_positions(), system.get_atomic_numbers()
+        cell = pbc = None
+        if hasattr(system, "get_cell"):
+            cl = system.get_cell()
+            cell = getattr(cl, "array", cl)
+        if hasattr(system, "get_pbc"):
+            pbc = system.get_pbc()
+    else:
+        raise TypeError("Unsupported system type. Provide ASE.Atoms-like, dict, or (positions, Z, cell, pbc).")
+    if keep_device and isinstance(pos, torch.Tensor) and pos.is_cuda:
+        pos = pos.detach().to(torch.float64).reshape(-1, 3).contiguous()
+    else:
+        pos = np.ascontiguousarray(_host(pos, np.float64).reshape(-1, 3))
+    Z = _host(Z, np.int64).reshape(-1)
+    cell = None if cell is None else _host(cell, np.float64).reshape(3, 3)
+    pbc = None if pbc is None else _host(pbc, bool).reshape(3)
+    return pos, Z, cell, pbc
+
+
+def linspace_ref(start: float, end: float, n: int) -> np.ndarray:
+    """torch.linspace in float64 (the reference builds its GTO widths with it, soap_generator.py:694).
+
+    step = (end - start) / (n - 1); below the midpoint fma(step, i, start), above it
+    fma(-step, n - 1 - i, end), each correctly rounded once (evaluated exactly here).
+    """
+    from fractions import Fraction
+    if n == 1:
+        return np.array([float(start)])
+    step = (end - start) / (n - 1)
+    half = n // 2
+    return np.array([float(Fraction(start) + Fraction(step) * i) if i < half
+                     else float(Fraction(end) - Fraction(step) * (n - 1 - i)) for i in range(n)])
+
+
+def gauss_legendre(n: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Golub-Welsch nodes and weights on [-1, 1] (soap_generator.py:350-362)."""
+    k = np.arange(1, n, dtype=np.float64)
+    beta = k / np.sqrt(4 * k * k - 1)
+    J = np.diag(beta, 1) + np.diag(beta, -1)
+    x, V = np.linalg.eigh(J)
+    return x, 2 * (V[0, :] ** 2)
+
+
+def perpendicular_lengths(cell: np.ndarray) -> np.ndarray:
+    """soap_generator.py:369-379."""
+    a, b, c = cell
+    v = float(np.dot(a, np.cross(b, c)))
+    return np.array([abs(v) / (np.linalg.norm(np.cross(b, c)) + 1e-12),
+                     abs(v) / (np.linalg.norm(np.cross(c, a)) + 1e-12),
+                     abs(v) / (np.linalg.norm(np.cross(a, b)) + 1e-12)])
+
+
+def pbc_image_shifts(cell: np.ndarray, pbc: np.ndarray, cutoff: float) -> np.ndarray:
+    """Cartesian image shifts in meshgrid-'ij' order over -n..n (soap_generator.py:382-396, 1016-1017)."""
+    h = perpendicular_lengths(cell)
+    n_rep = [int(math.ceil(cutoff / (h[i] + 1e-12))) if bool(pbc[i]) else 0 for i in range(3)]
+    ox, oy, oz = np.meshgrid(*[np.arange(-n, n + 1) for n in n_rep], indexing="ij")
+    o = np.stack([ox.ravel(), oy.ravel(), oz.ravel()], 1).astype(np.float64)
+    return np.ascontiguousarray((o[:, 0:1] * cell[0] + o[:, 1:2] * cell[1]) + o[:, 2:3] * cell[2])
+
+
+@dataclass
+class Weighting:
+    """DScribe-style weighting (soap_generator.py:403-470); evaluated on the device."""
+    function: Optional[str] = None
+    r0: float = 1.0
+    c: float = 1.0
+    d: float = 1.0
+    m: float = 1.0
+    w0: float = 1.0
+
+    def infer_rcut(self, threshold: float = 1e-2) -> float:
+        if self.function is None:
+            raise ValueError("Cannot infer r_cut when weighting.function is None; provide r_cut explicitly.")
+        if self.function == "poly":
+            return float(self.r0)
+        if self.function in ("pow", "exp"):
+            val = self.c / threshold - self.d
+            if val <= 0:
+                raise ValueError(f"Invalid weighting: c/threshold - d must be >0 for {self.function}.")
+            if self.function == "pow":
+                return float(self.r0 * (val ** (1.0 / self.m)))
+            return float(-self.r0 * math.log(val))
+        raise ValueError(f"Unknown weighting function: {self.function}")
+
+    @property
+    def code(self) -> int:
+        return {None: _lib.W_NONE, "poly": _lib.W_POLY, "pow": _lib.W_POW, "exp": _lib.W_EXP}[self.function]
+
+
+# ----------------------------------------------------------------------------
+# the SOAP class
+# ----------------------------------------------------------------------------
+
+_PRECISIONS = {"float64": _lib.PREC_F64, "mixed": _lib.PREC_MIXED, "float32": _lib.PREC_F32}
+
+
+class _NL:
+    """CSR neighbour list on the device: offsets (C+1) int64, nbr (E) int32 ext indices."""
+
+    def __init__(self, offsets, nbr, n_edges, max_count):
+        self.offsets, self.nbr, self.n_edges, self.max_count = offsets, nbr, n_edges, max_count
+
+
+class _Ctx:
+    """Device-resident inputs of one system."""
+
+
+class SOAP:
+    """DScribe-compatible SOAP on MI355X (signature of soap_generator.py:524-541).
+
+    Extra keyword ``precision`` selects the arithmetic of the coefficient stage:
+    "float64" (default, what the reference computes internally), "mixed"
+    (float32 harmonics, float64 radial functions and accumulation) or
+    "float32".  Output tensors are ``dtype`` as in the reference.
+    """
+
+    def __init__(self, r_cut=None, n_max=None, l_max=None, sigma=1.0, rbf="gto", weighting=None,
+                 crossover=True, average="off", species=None, periodic=False, sparse=False,
+                 dtype="float32", device=None, quad_n=100, max_num_neighbors=None,
+                 precision="float64", _betas=None):
+        if n_max is None or l_max is None:
+            raise ValueError("n_max and l_max must be provided.")
+        if species is None or len(species) == 0:
+            raise ValueError("species must be provided (list of atomic numbers or symbols).")
+        self._dtype_str = dtype
+        self.dtype = _torch_dtype_from_str(dtype)
+        self.device = torch.device(device) if device is not None else \
+            torch.device("cuda" if torch.cuda.is_available() else "cpu")
+        zs = []
+        for s in species:
+            if isinstance(s, str):
+                if s not in SYMBOLS:
+                    raise ValueError(f"Cannot resolve species symbol '{s}'. Use atomic numbers.")
+                zs.append(SYMBOLS[s])
+            else:
+                zs.append(int(s))
+        self._species_Z = sorted(set(zs))
+        self._atomic_number_set = set(self._species_Z)
+        self._Z_to_species_index = {z: i for i, z in enumerate(self._species_Z)}
+        self.n_species = len(self._species_Z)
+        self._n_max, self._l_max = int(n_max), int(l_max)
+        self._sigma = float(sigma)
+        self._eta = 1.0 / (2.0 * self._sigma * self._sigma)
+        self._rbf = str(rbf).lower()
+        self.crossover = bool(crossover)
+        self.average = str(average)
+        self.periodic = bool(periodic)
+        self.sparse = bool(sparse)
+        self.quad_n = int(quad_n)
+        self.max_num_neighbors = max_num_neighbors   # exhaustive search: accepted, no truncation
+        if precision not in _PRECISIONS:
+            raise ValueError("precision must be 'float64', 'mixed' or 'float32'")
+        self.precision = precision
+        if self.average not in ("off", "inner", "outer", "cc"):
+            raise ValueError("average must be one of: 'off', 'inner', 'outer', 'cc' (cc = projection coeffs).")
+        self._weighting_dict = dict(weighting) if weighting is not None else None
+        thr = 1e-2
+        if self._weighting_dict is not None:
+            thr = float(self._weighting_dict.get("threshold", 1e-2))
+            self._weighting = Weighting(**{k: v for k, v in self._weighting_dict.items() if k != "threshold"})
+        else:
+            self._weighting = None
+        if r_cut is None:
+            if self._weighting is None:
+                raise ValueError("Either r_cut or weighting must be provided.")
+            self._r_cut = float(self._weighting.infer_rcut(threshold=thr))
+        else:
+            self._r_cut = float(r_cut)
+        self._cutoff_padding = float(self._sigma * math.sqrt(-2.0 * math.log(1e-3)))
+        self._cutoff = self._r_cut + self._cutoff_padding
+        if self._n_max > 16 or self._l_max > 9:
+            raise NotImplementedError("cusoap_amd kernels are compiled for n_max <= 16 and l_max <= 9")
+        if self._rbf not in ("gto", "polynomial"):
+            raise ValueError("rbf must be 'gto' or 'polynomial'.")
+        self._on_gpu = self.device.type == "cuda" and torch.cuda.is_available()
+        if self._rbf == "gto":
+            self._init_gto_basis(_betas)
+        else:
+            self._init_poly_basis()
+        self._init_self_contributions()
+        self.n_features = int(self.get_number_of_features())
+        self._build_feature_tables()
+        self._basis_struct = None
+        self._last_nl_backend = "hip-cell-list"
+
+    # ---- basis (soap_generator.py:690-819) ---------------------------------
+    def _init_gto_basis(self, betas=None):
+        a = linspace_ref(1.0, self._r_cut, self._n_max)
+        alphas, S = [], []
+        for l in range(self._l_max + 1):
+            alpha_l = -np.log(1e-3 / (a ** l)) / (a ** 2)
+            m = alpha_l[:, None] + alpha_l[None, :]
+            S.append(0.5 * math.exp(math.lgamma(l + 1.5)) * (m ** (-(l + 1.5))))
+            alphas.append(alpha_l)
+        self._alphas = np.stack(alphas)
+        S = np.ascontiguousarray(np.stack(S))
+        if betas is not None:
+            self._betas = np.asarray(betas, dtype=np.float64)
+        elif self._on_gpu:
+            self._betas = self._lowdin_gpu(S)
+        else:
+            w, V = np.linalg.eigh(S)
+            w = np.maximum(w, 1e-12)
+            self._betas = np.einsum("lik,lk,ljk->lij", V, 1.0 / np.sqrt(w), V)
+        p = self._alphas + self._eta
+        self._gto_gamma = self._alphas * self._eta / p
+        lv = np.arange(self._l_max + 1, dtype=np.float64)[:, None]
+        self._gto_pref = (PI ** 1.5) * (self._eta / p) ** lv * p ** -1.5
+
+    def _lowdin_gpu(self, S: np.ndarray) -> np.ndarray:
+        """Loewdin S^-1/2 in float64 on the device (replaces soap_generator.py:339-343)."""
+        lib = _lib.load()
+        Sd = torch.from_numpy(S).to(self.device)
+        out = torch.empty_like(Sd)
+        _lib.check(lib.cusoap_lowdin_invsqrt_f64(Sd.data_ptr(), S.shape[0], S.shape[1], 1e-12, out.data_ptr(),
+                                                 self._stream()), "lowdin_invsqrt_f64")
+        return out.cpu().numpy()
+
+    def _init_poly_basis(self):
+        """soap_generator.py:710-761: beta = scipy sqrtm(inv(S)) on the host, GL(quad_n) grid."""
+        from scipy.linalg import sqrtm
+        n, rc = self._n_max, self._r_cut
+        S = np.zeros((n, n))
+        for i in range(1, n + 1):
+            for j in range(1, n + 1):
+                S[i - 1, j - 1] = (2 * rc ** (7 + i + j)) / ((5 + i + j) * (6 + i + j) * (7 + i + j))
+        betas = sqrtm(np.linalg.inv(S))
+        if np.iscomplexobj(betas):
+            raise ValueError("Could not calculate normalization factors for the radial basis in the domain of "
+                             "real numbers. Lowering the number of radial basis functions (n_max) or increasing "
+                             "the radial cutoff (r_cut) is advised.")
+        x, w = gauss_legendre(self.quad_n)
+        self._rx = rc * 0.5 * (x + 1.0)
+        self._wr = rc * 0.5 * w
+        rclip = np.clip(self._rx, 0.0, rc)
+        fs = np.stack([(rc - rclip) ** (k + 2) for k in range(1, n + 1)])
+        self._gss = np.ascontiguousarray(betas) @ fs
+        # r -> 0 limits of B and Bhat (soap_generator.py:1509-1546)
+        eta, rx, wr = self._eta, self._rx, self._wr
+        common0 = (4.0 * PI) * rx * rx * wr * np.exp(-eta * rx * rx)
+        L1 = self._l_max + 1
+        B0, Bh0 = np.empty((n, L1)), np.empty((n, L1))
+        fl, df = np.ones_like(rx), 1.0
+        for l in range(L1):
+            dfn = df * (2 * l + 3)
+            B0[:, l] = (common0 * fl / df) @ self._gss.T
+            Bh0[:, l] = (2.0 * eta) * ((common0 * fl * (2.0 * eta) * rx * rx / dfn) @ self._gss.T - B0[:, l])
+            fl = fl * (2.0 * eta) * rx
+            df = dfn
+        self._poly_B0, self._poly_Bh0 = B0, Bh0
+
+    def _init_self_contributions(self):
+        """l=0 self coefficient of a Gaussian at the center (soap_generator.py:765-795)."""
+        w0 = float(self._weighting.w0) if self._weighting is not None else 1.0
+        if self._rbf == "gto":
+            p = self._alphas[0] + self._eta
+            self._self_l0 = (self._betas[0] @ (w0 * (PI ** 1.5) * p ** -1.5)) * Y00
+        else:
+            common0 = (4.0 * PI) * (self._rx * self._rx) * np.exp(-self._eta * self._rx ** 2) * self._wr
+            self._self_l0 = w0 * (common0 @ self._gss.T) * Y00
+
+    # ---- features (soap_generator.py:908-962, 1215-1291) -----------------------
+    def get_number_of_features(self) -> int:
+        if self.average == "cc":
+            return self.n_species * self._n_max * (self._l_max + 1) ** 2
+        return self._n_power_features()
+
+    def _n_power_features(self) -> int:
+        if self.crossover:
+            ch = self.n_species * self._n_max
+            return int((ch * (ch + 1) // 2) * (self._l_max + 1))
+        return int(self.n_species * (self._l_max + 1) * (self._n_max * (self._n_max + 1) // 2))
+
+    def _build_feature_tables(self):
+        n, S, L1 = self._n_max, self.n_species, self._l_max + 1
+        rows = []
+        for j in range(S):
+            for jd in range(j, S if self.crossover else j + 1):
+                for l in range(L1):
+                    for a in range(n):
+                        for b in range(a if j == jd else 0, n):
+                            rows.append((j, jd, a, b, l))
+        ft = np.array(rows, dtype=np.int64).reshape(-1, 5)
+        if ft.shape[0] != self._n_power_features():
+            raise RuntimeError("Feature table size mismatch.")
+        self._feat_rows = ft
+        self._feat_packed = (ft[:, 0] | (ft[:, 1] << 8) | (ft[:, 2] << 16) | (ft[:, 3] << 22)
+                             | (ft[:, 4] << 28)).astype(np.uint32)
+        SN = S * n
+        pf = -np.ones((L1, SN, SN), dtype=np.int32)
+        for f, (j, jd, a, b, l) in enumerate(ft):
+            pf[l, j * n + a, jd * n + b] = f
+            pf[l, jd * n + b, j * n + a] = f
+        self._pair_feature = pf
+
+    def get_location(self, species_pair) -> slice:
+        """DScribe-style feature slice of a species pair (soap_generator.py:921-962)."""
+        if self.average == "cc":
+            raise ValueError("get_location is not defined for average='cc' (projection coefficients).")
+        if len(species_pair) != 2:
+            raise ValueError("species_pair must be length-2.")
+
+        def toZ(x):
+            if isinstance(x, str):
+                if x not in SYMBOLS:
+                    raise ValueError(f"Cannot resolve species symbol '{x}'.")
+                return SYMBOLS[x]
+            return int(x)
+
+        Z1, Z2 = toZ(species_pair[0]), toZ(species_pair[1])
+        if Z1 not in self._atomic_number_set or Z2 not in self._atomic_number_set:
+            raise ValueError("Species not present in descriptor species list.")
+        i, j = self._Z_to_species_index[Z1], self._Z_to_species_index[Z2]
+        if i > j:
+            i, j = j, i
+        symm = (self._n_max * (self._n_max + 1) // 2) * (self._l_max + 1)
+        unsymm = (self._n_max * self._n_max) * (self._l_max + 1)
+        if self.crossover:
+            m_symm = i + int(j > i)
+            m_unsymm = j + i * self.n_species - i * (i + 1) / 2 - m_symm
+            start = int(m_symm * symm + m_unsymm * unsymm)
+            return slice(start, start + (symm if i == j else unsymm))
+        if i != j:
+            raise ValueError("crossover=False: no cross-species blocks exist.")
+        return slice(i * symm, i * symm + symm)
+
+    # ---- device plumbing ------------------------------------------------------
+    def _require_gpu(self):
+        if not self._on_gpu:
+            raise RuntimeError("cusoap_amd runs on a HIP GPU only (device=%s, cuda available=%s); there is no CPU path"
+                               % (self.device, torch.cuda.is_available()))
+        _lib.load()
+
+    def _stream(self) -> int:
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _dev(self, a: np.ndarray, dtype=None) -> torch.Tensor:
+        t = torch.from_numpy(np.ascontiguousarray(a))
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.to(self.device, non_blocking=False)
+
+    def _basis(self) -> "_lib.Basis":
+        if self._basis_struct is not None:
+            return self._basis_struct
+        self._require_gpu()
+        b = _lib.Basis()
+        b.n_species, b.n_max, b.l_max = self.n_species, self._n_max, self._l_max
+        b.rbf = _lib.RBF_GTO if self._rbf == "gto" else _lib.RBF_POLY
+        b.eta, b.cutoff = self._eta, self._cutoff
+        w = self._weighting
+        b.weighting = _lib.W_NONE if w is None else w.code
+        if w is not None:
+            b.w_r0, b.w_c, b.w_d, b.w_m, b.w0 = w.r0, w.c, w.d, w.m, w.w0
+        else:
+            b.w_r0, b.w_c, b.w_d, b.w_m, b.w0 = 1.0, 1.0, 1.0, 1.0, 1.0
+        keep = {}
+        if self._rbf == "gto":
+            keep["gamma"] = self._dev(self._gto_gamma)
+            keep["pref"] = self._dev(self._gto_pref)
+            keep["beta"] = self._dev(self._betas)
+            b.gto_gamma, b.gto_pref, b.gto_beta = (keep[k].data_ptr() for k in ("gamma", "pref", "beta"))
+            b.quad_n = 0
+        else:
+            keep["rx"] = self._dev(self._rx)
+            keep["rxw"] = self._dev((4.0 * PI) * (self._rx * self._rx * self._wr))
+            keep["gss"] = self._dev(self._gss)
+            keep["B0"] = self._dev(self._poly_B0)
+            keep["Bh0"] = self._dev(self._poly_Bh0)
+            b.poly_rx, b.poly_rxw, b.poly_gss = keep["rx"].data_ptr(), keep["rxw"].data_ptr(), keep["gss"].data_ptr()
+            b.poly_B0, b.poly_Bh0 = keep["B0"].data_ptr(), keep["Bh0"].data_ptr()
+            b.quad_n = self.quad_n
+        keep["self_l0"] = self._dev(self._self_l0)
+        b.self_l0 = keep["self_l0"].data_ptr()
+        keep["ft"] = self._dev(self._feat_packed.view(np.int32))
+        keep["pf"] = self._dev(self._pair_feature)
+        self._keep = keep
+        self._basis_struct = b
+        return b
+
+    # ---- centers (soap_generator.py:966-1013) ----------------------------------
+    def prepare_centers(self, positions: np.ndarray, centers) -> Tuple[np.ndarray, np.ndarray]:
+        """Centers as (C,3) float64 and the atom index of each (or -1); coordinates snap within 1e-12."""
+        if centers is None:
+            return positions, np.arange(positions.shape[0], dtype=np.int64)
+        centers = list(centers)
+        if len(centers) == 0:
+            raise RuntimeError("stack expects a non-empty TensorList")
+        xyz = np.empty((len(centers), 3))
+        idx = np.empty(len(centers), dtype=np.int64)
+        for k, c in enumerate(centers):
+            if isinstance(c, int):
+                xyz[k] = positions[c]
+                idx[k] = int(c)
+            else:
+                cc = _host(c, np.float64)
+                if cc.size != 3:
+                    raise RuntimeError(f"shape '[3]' is invalid for input of size {cc.size}")
+                xyz[k] = cc.reshape(3)
+                idx[k] = -1
+                if positions.shape[0] > 0:
+                    d2 = ((positions - xyz[k][None, :]) ** 2).sum(1)
+                    j = int(np.argmin(d2))
+                    if d2[j] <= 1e-12 * 1e-12:
+                        idx[k] = j
+        return xyz, idx
+
+    def _context(self, system, centers) -> _Ctx:
+        self._require_gpu()
+        pos, Z, cell, pbc = system_to_arrays(system, keep_device=True)
+        bad = set(np.unique(Z).tolist()) - self._atomic_number_set
+        if bad:
+            raise ValueError(f"System contains atomic numbers not in species list: {sorted(bad)}")
+        x = _Ctx()
+        if isinstance(pos, torch.Tensor):
+            x.pos = pos.to(self.device)
+            x.n_atoms = pos.shape[0]
+        else:
+            x.pos = self._dev(pos)
+            x.n_atoms = pos.shape[0]
+        if centers is None:
+            x.centers = x.pos
+            cidx = np.arange(x.n_atoms, dtype=np.int64)
+        else:
+            pos_h = pos.cpu().numpy() if isinstance(pos, torch.Tensor) else pos
+            cxyz, cidx = self.prepare_centers(pos_h, centers)
+            x.centers = self._dev(np.ascontiguousarray(cxyz))
+        if self.periodic:
+            if cell is None or pbc is None:
+                raise ValueError("periodic=True requires cell and pbc.")
+            shifts = pbc_image_shifts(cell, pbc, self._cutoff)
+        else:
+            shifts = np.zeros((1, 3))
+        lut = np.full(max(self._species_Z + [int(Z.max()) if Z.size else 0]) + 1, -1, dtype=np.int32)
+        for z, i in self._Z_to_species_index.items():
+            lut[z] = i
+        sp = lut[Z] if Z.size else np.zeros(0, np.int32)
+        x.n_centers, x.n_img = int(cidx.shape[0]), shifts.shape[0]
+        x.Z_h, x.cidx_h, x.sp_h = Z, cidx, sp
+        x.sp = self._dev(sp.astype(np.int32))
+        x.shifts = self._dev(shifts)
+        self_sp = np.where(cidx >= 0, sp[np.maximum(cidx, 0)] if sp.size else -1, -1).astype(np.int32)
+        x.self_sp = self._dev(self_sp)
+        return x
+
+    # ---- pipeline stages ----------------------------------------------------------
+    def _neighbors(self, x: _Ctx, drop_self: bool) -> _NL:
+        lib = _lib.load()
+        st = self._stream()
+        plan = _lib.NLPlan()
+        tmp = torch.empty(16384, dtype=torch.uint8, device=self.device)
+        _lib.check(lib.cusoap_nl_plan(x.centers.data_ptr(), x.n_centers, x.n_atoms, x.n_img, self._cutoff,
+                                      tmp.data_ptr(), C.byref(plan), st), "nl_plan")
+        ws = torch.empty(int(plan.ws_bytes), dtype=torch.uint8, device=self.device)
+        offsets = torch.empty(x.n_centers + 1, dtype=torch.int64, device=self.device)
+        total, mx = C.c_int64(0), C.c_int32(0)
+        _lib.check(lib.cusoap_nl_count(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
+                                       x.centers.data_ptr(), x.n_centers, self._cutoff, int(drop_self),
+                                       ws.data_ptr(), ws.numel(), offsets.data_ptr(), C.byref(total), C.byref(mx),
+                                       st), "nl_count")
+        nbr = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=self.device)
+        _lib.check(lib.cusoap_nl_fill(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
+                                      x.centers.data_ptr(), x.n_centers, self._cutoff, int(drop_self),
+                                      ws.data_ptr(), ws.numel(), offsets.data_ptr(), mx.value, nbr.data_ptr(), st),
+                   "nl_fill")
+        return _NL(offsets, nbr, int(total.value), int(mx.value))
+
+    def _coeff_dtype(self, precision: str) -> torch.dtype:
+        return torch.float32 if precision == "float32" else torch.float64
+
+    def _coefficients(self, x: _Ctx, nl: _NL, create_path: bool, precision: Optional[str] = None) -> torch.Tensor:
+        precision = precision or self.precision
+        lib = _lib.load()
+        b = self._basis()
+        L2 = (self._l_max + 1) ** 2
+        out = torch.empty((x.n_centers, self.n_species, self._n_max, L2), dtype=self._coeff_dtype(precision),
+                          device=self.device)
+        _lib.check(lib.cusoap_coefficients(C.byref(b), _PRECISIONS[precision], x.pos.data_ptr(), x.n_atoms,
+                                           x.sp.data_ptr(), x.shifts.data_ptr(), x.n_img, x.centers.data_ptr(),
+                                           x.n_centers, x.self_sp.data_ptr(), nl.offsets.data_ptr(),
+                                           nl.nbr.data_ptr(), int(create_path), out.data_ptr(), self._stream()),
+                   "coefficients")
+        return out
+
+    def _descriptor(self, coeffs: torch.Tensor, average: Optional[str] = None) -> torch.Tensor:
+        avg = self.average if average is None else average
+        lib = _lib.load()
+        b = self._basis()
+        cdt = _lib.DT_F64 if coeffs.dtype == torch.float64 else _lib.DT_F32
+        odt = _lib.DT_F64 if self.dtype == torch.float64 else _lib.DT_F32
+        C_ = coeffs.shape[0]
+        if avg == "cc":
+            out = torch.empty((C_, self.n_species * self._n_max * (self._l_max + 1) ** 2), dtype=self.dtype,
+                              device=self.device)
+            _lib.check(lib.cusoap_cc_projection(C.byref(b), coeffs.data_ptr(), cdt, C_, out.data_ptr(), odt,
+                                                self._stream()), "cc_projection")
+            return out
+        nf = self._n_power_features()
+        out = torch.empty((C_, nf), dtype=self.dtype, device=self.device)
+        _lib.check(lib.cusoap_power_spectrum(C.byref(b), coeffs.data_ptr(), cdt, C_, self._keep["ft"].data_ptr(),
+                                             nf, out.data_ptr(), odt, self._stream()), "power_spectrum")
+        return out
+
+    # ---- public forward API (soap_generator.py:1691-1810) -----------------------
+    def generate(self, system, centers=None, n_jobs: int = 1, only_physical_cores: bool = False,
+                 verbose: bool = False):
+        if isinstance(system, list):
+            systems = system
+            if centers is None:
+                centers_list = [None] * len(systems)
+            else:
+                if not isinstance(centers, list):
+                    raise ValueError("For multiple systems, centers must be a list with one entry per system.")
+                centers_list = centers
+                if len(centers_list) != len(systems):
+                    raise ValueError("centers list must match number of systems.")
+            outs = [self.generate(s, c) for s, c in zip(systems, centers_list)]
+            try:
+                return torch.stack(outs, dim=0)
+            except Exception:
+                return outs
+        out = self._evaluate(system, centers)
+        return out.to_sparse_coo() if self.sparse else out
+
+    def create(self, system, centers=None, n_jobs: int = 1, only_physical_cores: bool = False,
+               verbose: bool = False):
+        return self.generate(system, centers, n_jobs=n_jobs, only_physical_cores=only_physical_cores,
+                             verbose=verbose)
+
+    def _evaluate(self, system, centers=None, average: Optional[str] = None) -> torch.Tensor:
+        x = self._context(system, centers)
+        nl = self._neighbors(x, drop_self=True)
+        coeffs = self._coefficients(x, nl, create_path=True)
+        return self._descriptor(coeffs, average)
+
+    def coefficients(self, system, centers=None, create_path: bool = True) -> torch.Tensor:
+        """c[center, species, n, lm] (float64 unless precision='float32')."""
+        x = self._context(system, centers)
+        nl = self._neighbors(x, drop_self=create_path)
+        return self._coefficients(x, nl, create_path=create_path)
+
+    def neighbor_list(self, system, centers=None, drop_self: bool = True):
+        """(offsets, ext_indices) of the device neighbour list, as int64 / int32 tensors."""
+        x = self._context(system, centers)
+        nl = self._neighbors(x, drop_self=drop_self)
+        return nl.offsets, nl.nbr[:nl.n_edges]
+
+    @torch.no_grad()
+    def create_with_profile(self, system, centers=None):
+        """Descriptor, per-stage seconds and run statistics (soap_generator.py:1815-1868)."""
+        times: Dict[str, float] = {}
+
+        def mark(name, t0):
+            torch.cuda.synchronize(self.device)
+            times[name] = time.perf_counter() - t0
+            return time.perf_counter()
+
+        torch.cuda.synchronize(self.device)
+        t_all = t = time.perf_counter()
+        x = self._context(system, centers)
+        t = mark("to_tensors", t)
+        nl = self._neighbors(x, drop_self=True)
+        t = mark("neighbor", t)
+        coeffs = self._coefficients(x, nl, create_path=True)
+        t = mark("coefficients", t)
+        out = self._descriptor(coeffs)
+        t = mark("power_spectrum", t)
+        times["total"] = time.perf_counter() - t_all
+        extra = {"n_atoms": x.n_atoms, "n_centers": x.n_centers, "E_edges": nl.n_edges,
+                 "neighbors_per_center": nl.n_edges / x.n_centers if x.n_centers else 0.0,
+                 "nl_backend": self._last_nl_backend}
+        return (out.to_sparse_coo() if self.sparse else out), times, extra
+
+    # ---- derivatives (soap_generator.py:1871-2699) ------------------------------------
+    @staticmethod
+    def _included(n_atoms, include, exclude) -> List[int]:
+        idx = list(range(n_atoms)) if include is None else [int(i) for i in include]
+        if exclude is not None:
+            ex = set(int(i) for i in exclude)
+            idx = [i for i in idx if i not in ex]
+        return idx
+
+    def _attach_array(self, x: _Ctx, centers, attach: bool) -> Optional[torch.Tensor]:
+        if not attach:
+            return None
+        if centers is None:
+            a = np.arange(x.n_centers, dtype=np.int32)
+        else:
+            a = np.array([int(c) if isinstance(c, int) else -1 for c in centers], dtype=np.int32)
+        return self._dev(a)
+
+    def _dense_jacobian(self, system, centers, include, exclude, attach, return_descriptor):
+        x = self._context(system, centers)
+        lib = _lib.load()
+        b = self._basis()
+        idx = self._included(x.n_atoms, include, exclude)
+        n_inc = len(idx)
+        inv = -np.ones(x.n_atoms, dtype=np.int32)
+        if n_inc:
+            inv[np.asarray(idx)] = np.arange(n_inc, dtype=np.int32)
+        closed = (not self.periodic) and self._weighting is None and not attach
+        nl_keep = self._neighbors(x, drop_self=False)
+        prec = "float64" if self.precision == "float32" else self.precision
+        if closed:
+            coeffs = self._coefficients(x, nl_keep, create_path=False, precision=prec)
+        else:
+            nl_drop = self._neighbors(x, drop_self=True)
+            coeffs = self._coefficients(x, nl_drop, create_path=True, precision=prec)
+        mode = _lib.MODE_CC if self.average == "cc" else _lib.MODE_PS
+        n_feat = self.get_number_of_features()
+        deriv = torch.empty((x.n_centers, n_inc, 3, n_feat), dtype=self.dtype, device=self.device)
+        nbytes = C.c_size_t(0)
+        _lib.check(lib.cusoap_jacobian_workspace_size(C.byref(b), mode, nl_keep.n_edges, C.byref(nbytes)),
+                   "jacobian_workspace_size")
+        ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
+        inv_d = self._dev(inv)
+        att = self._attach_array(x, centers, attach)
+        odt = _lib.DT_F64 if self.dtype == torch.float64 else _lib.DT_F32
+        if deriv.numel() > 0:
+            _lib.check(lib.cusoap_jacobian_dense(
+                C.byref(b), mode, x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(), x.shifts.data_ptr(), x.n_img,
+                x.centers.data_ptr(), x.n_centers, att.data_ptr() if att is not None else None,
+                nl_keep.offsets.data_ptr(), nl_keep.nbr.data_ptr(), nl_keep.n_edges, coeffs.data_ptr(),
+                inv_d.data_ptr(), n_inc, self._keep["ft"].data_ptr(), n_feat, ws.data_ptr(), ws.numel(),
+                deriv.data_ptr(), odt, self._stream()), "jacobian_dense")
+        if return_descriptor:
+            return deriv, self._descriptor(coeffs)
+        return deriv
+
+    def derivatives_analytical_ps(self, system, centers=None, include=None, exclude=None,
+                                  return_descriptor=True):
+        """Closed-form Jacobian, fixed centers (soap_generator.py:2270-2626)."""
+        if self._rbf not in ("gto", "polynomial"):
+            raise NotImplementedError("closed-form ps derivative needs rbf='gto' or 'polynomial'; use derivatives_analytical.")
+        if self.periodic:
+            raise NotImplementedError("closed-form ps derivative is non-periodic; use derivatives_analytical.")
+        if self._weighting is not None:
+            raise NotImplementedError("closed-form ps derivative assumes weighting=None; use derivatives_analytical.")
+        return self._dense_jacobian(system, centers, include, exclude, False, return_descriptor)
+
+    def derivatives_analytical(self, system, centers=None, include=None, exclude=None,
+                               return_descriptor=True, attach=False):
+        """Exact Jacobian for every configuration (the reference's autograd path,
+        soap_generator.py:2007-2113), here in closed form on the device."""
+        return self._dense_jacobian(system, centers, include, exclude, attach, return_descriptor)
+
+    def derivatives_numerical(self, system, centers=None, include=None, exclude=None, method="auto",
+                              return_descriptor=True, attach=False, n_jobs=1, only_physical_cores=False,
+                              verbose=False, d=1e-4):
+        """Central differences with fixed centers, float64 evaluations (soap_generator.py:1871-2003)."""
+        if method not in ("auto", "numerical"):
+            raise ValueError("method must be 'auto' or 'numerical'.")
+        desc0 = self.create(system, centers) if return_descriptor else None
+        pos, Z, cell, pbc = system_to_arrays(system)
+        idx = self._included(pos.shape[0], include, exclude)
+        cxyz, _ = self.prepare_centers(pos, centers)
+        base = [cxyz[k].copy() for k in range(cxyz.shape[0])]
+        if attach:
+            attached = list(range(cxyz.shape[0])) if centers is None else \
+                [int(c) if isinstance(c, int) else -1 for c in centers]
+        else:
+            attached = [-1] * cxyz.shape[0]
+        orig = self.dtype
+        self.dtype = torch.float64
+        try:
+            deriv = torch.zeros((cxyz.shape[0], len(idx), 3, self.get_number_of_features()),
+                                dtype=torch.float64, device=self.device)
+            for ai, a in enumerate(idx):
+                for comp in range(3):
+                    fs = []
+                    for sgn in (1.0, -1.0):
+                        p = pos.copy()
+                        p[a, comp] += sgn * d
+                        sysd = {"positions": p, "atomic_numbers": Z, "cell": cell, "pbc": pbc}
+                        cen = [p[t] if t == a else b for t, b in zip(attached, base)]
+                        fs.append(self._evaluate(sysd, cen))
+                    deriv[:, ai, comp, :] = (fs[0] - fs[1]) / (2.0 * d)
+        finally:
+            self.dtype = orig
+        deriv = deriv.to(orig)
+        return (deriv, desc0) if return_descriptor else deriv
+
+    def derivatives(self, system, centers=None, include=None, exclude=None, method="auto",
+                    return_descriptor=True, attach=False, n_jobs=1, only_physical_cores=False, verbose=False):
+        """Dispatch as the reference (soap_generator.py:2628-2699)."""
+        if method == "numerical":
+            return self.derivatives_numerical(system=system, centers=centers, include=include, exclude=exclude,
+                                              method="numerical", return_descriptor=return_descriptor,
+                                              attach=attach)
+        if method in ("auto", "analytical"):
+            if (not attach and not self.periodic and self._weighting is None
+                    and self.average in ("off", "outer", "cc", "inner")):
+                return self.derivatives_analytical_ps(system=system, centers=centers, include=include,
+                                                      exclude=exclude, return_descriptor=return_descriptor)
+            return self.derivatives_analytical(system=system, centers=centers, include=include, exclude=exclude,
+                                               return_descriptor=return_descriptor, attach=attach)
+        raise ValueError("method must be one of: 'auto', 'analytical', 'numerical'.")
+
+    # ---- adjoint (no reference counterpart) ------------------------------------------------
+    def vjp(self, system, grad_desc, centers=None, attach=False) -> torch.Tensor:
+        """sum_{c,f} grad_desc[c,f] d desc[c,f] / d positions, (n_atoms, 3) float64, without dp/dR.
+
+        grad_desc: (n_centers, n_features) or a single (n_features) row shared by
+        every center (a linear readout).  Fixed centers unless attach=True.
+        """
+        x = self._context(system, centers)
+        return self._vjp_ctx(x, grad_desc, centers, attach)
+
+    def _vjp_ctx(self, x: _Ctx, grad_desc, centers, attach, coeffs=None, nl_keep=None):
+        lib = _lib.load()
+        b = self._basis()
+        n_feat = self.get_number_of_features()
+        g = grad_desc if isinstance(grad_desc, torch.Tensor) else torch.as_tensor(np.asarray(grad_desc))
+        g = g.to(self.device)
+        if g.dtype not in (torch.float32, torch.float64):
+            g = g.to(torch.float64)
+        g = g.contiguous()
+        if g.dim() == 1:
+            if g.numel() != n_feat:
+                raise ValueError("grad_desc row must have n_features entries")
+            bcast = 1
+        else:
+            if tuple(g.shape) != (x.n_centers, n_feat):
+                raise ValueError(f"grad_desc must be ({x.n_centers}, {n_feat})")
+            bcast = 0
+        if nl_keep is None:
+            nl_keep = self._neighbors(x, drop_self=False)
+        if coeffs is None:
+            closed = (not self.periodic) and self._weighting is None
+            prec = "float64" if self.precision == "float32" else self.precision
+            if closed:
+                coeffs = self._coefficients(x, nl_keep, create_path=False, precision=prec)
+            else:
+                coeffs = self._coefficients(x, self._neighbors(x, drop_self=True), create_path=True, precision=prec)
+        if coeffs.dtype != torch.float64:
+            coeffs = coeffs.double()
+        nbytes = C.c_size_t(0)
+        _lib.check(lib.cusoap_vjp_workspace_size(C.byref(b), x.n_centers, C.byref(nbytes)), "vjp_workspace_size")
+        ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
+        grad = torch.empty((x.n_atoms, 3), dtype=torch.float64, device=self.device)
+        att = self._attach_array(x, centers, attach)
+        mode = _lib.MODE_CC if self.average == "cc" else _lib.MODE_PS
+        _lib.check(lib.cusoap_vjp(
+            C.byref(b), mode, x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(), x.shifts.data_ptr(), x.n_img,
+            x.centers.data_ptr(), x.n_centers, att.data_ptr() if att is not None else None,
+            nl_keep.offsets.data_ptr(), nl_keep.nbr.data_ptr(), coeffs.data_ptr(), self._keep["pf"].data_ptr(),
+            g.data_ptr(), _lib.DT_F64 if g.dtype == torch.float64 else _lib.DT_F32, bcast, n_feat,
+            ws.data_ptr(), ws.numel(), grad.data_ptr(), self._stream()), "vjp")
+        return grad
+
+    def descriptor_and_vjp(self, system, grad_desc, centers=None, attach=False):
+        """One neighbour search and coefficient pass for both the descriptor and its adjoint."""
+        x = self._context(system, centers)
+        nl_keep = self._neighbors(x, drop_self=False)
+        closed = (not self.periodic) and self._weighting is None
+        if closed:
+            coeffs = self._coefficients(x, nl_keep, create_path=False)
+        else:
+            coeffs = self._coefficients(x, self._neighbors(x, drop_self=True), create_path=True)
+        desc = self._descriptor(coeffs)
+        return desc, self._vjp_ctx(x, grad_desc, centers, attach, coeffs=coeffs, nl_keep=nl_keep)
+
+    def create_autograd(self, positions: torch.Tensor, atomic_numbers, cell=None, pbc=None, centers=None,
+                        attach=False) -> torch.Tensor:
+        """Descriptor as a differentiable function of ``positions`` (backward = the adjoint kernel)."""
+        return _SOAPFunction.apply(positions, self, atomic_numbers, cell, pbc, centers, attach)
+
+
+class _SOAPFunction(torch.autograd.Function):
+    """desc = SOAP(positions); backward through cusoap_vjp (no dp/dR is formed)."""
+
+    @staticmethod
+    def forward(ctx, positions, soap, Z, cell, pbc, centers, attach):
+        system = {"positions": positions.detach(), "atomic_numbers": Z, "cell": cell, "pbc": pbc}
+        desc = soap.create(system, centers)
+        ctx.soap, ctx.system, ctx.centers, ctx.attach = soap, system, centers, attach
+        ctx.pos_dtype, ctx.pos_device = positions.dtype, positions.device
+        return desc
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        g = ctx.soap.vjp(ctx.system, grad_out.contiguous(), ctx.centers, ctx.attach)
+        return g.to(device=ctx.pos_device, dtype=ctx.pos_dtype), None, None, None, None, None, None

@@ -1,0 +1,199 @@
+/*
+ * cusoap_amd -- C ABI of the MI355X (gfx950) SOAP descriptor kernels.
+ *
+ * The reference (TACC/cuSOAP) has no C/FFI boundary: its hot path is the
+ * Python class `SOAP` in src/cusoap/soap_generator.py.  These entry points
+ * replace the private stages of that class; the Python mirror
+ * `cusoap_amd.SOAP` keeps the public signatures and binds these functions
+ * with ctypes.  Each declaration names the reference code it replaces.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers allocated by the caller
+ *    (caller-owned memory, typically PyTorch's caching allocator), except
+ *    the `*_host` out-parameters and the plan/basis structs, which live in
+ *    host memory.
+ *  - `stream` is a hipStream_t passed as void*; all work is enqueued on it.
+ *    Functions marked "synchronises" wait for the stream before returning.
+ *  - Every function returns CUSOAP_OK or an error code; the text of the last
+ *    error of the calling thread is returned by cusoap_last_error().
+ *  - Real spherical harmonics use the l-major, m = -l..l order with positive
+ *    real harmonics (Y_1,-1..1 proportional to y, z, x).
+ *  - Coefficient arrays are c[center][species][n][lm], lm = l*l + l + m.
+ */
+#ifndef CUSOAP_AMD_H
+#define CUSOAP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CUSOAP_OK = 0,
+    CUSOAP_ERR_ARG = 1,          /* invalid argument (maps to ValueError)              */
+    CUSOAP_ERR_HIP = 2,          /* HIP runtime error (RuntimeError)                   */
+    CUSOAP_ERR_UNSUPPORTED = 3,  /* parameter outside the compiled range (NotImplemented) */
+    CUSOAP_ERR_CAPACITY = 4,     /* a per-center neighbour count exceeds a kernel limit  */
+    CUSOAP_ERR_WORKSPACE = 5     /* workspace smaller than the size query asked for      */
+};
+
+enum { CUSOAP_RBF_GTO = 0, CUSOAP_RBF_POLY = 1 };
+enum { CUSOAP_W_NONE = 0, CUSOAP_W_POLY = 1, CUSOAP_W_POW = 2, CUSOAP_W_EXP = 3 };
+/* Arithmetic of the coefficient stage.  F64: everything in double.  MIXED:
+ * harmonics in float, radial functions / accumulation / beta contraction in
+ * double.  F32: everything in float (the Loewdin contraction then loses
+ * digits at large n_max). */
+enum { CUSOAP_PREC_F64 = 0, CUSOAP_PREC_F32 = 1, CUSOAP_PREC_MIXED = 2 };
+enum { CUSOAP_DT_F64 = 0, CUSOAP_DT_F32 = 1 };
+enum { CUSOAP_MODE_PS = 0, CUSOAP_MODE_CC = 1 };
+
+/* Radial basis tables, all device pointers to float64.  Mirrors the state the
+ * reference builds in SOAP.__init__ (soap_generator.py:600-634, 690-819). */
+typedef struct cusoap_basis {
+    int32_t n_species, n_max, l_max, rbf;
+    double eta;                 /* 1 / (2 sigma^2)                                  */
+    double cutoff;              /* r_cut + sigma sqrt(-2 ln 1e-3)                   */
+    int32_t weighting;          /* CUSOAP_W_*                                       */
+    int32_t quad_n;             /* polynomial basis: Gauss-Legendre points          */
+    double w_r0, w_c, w_d, w_m, w0;
+    const double* gto_gamma;    /* (l_max+1, n_max)  alpha eta / (alpha + eta)      */
+    const double* gto_pref;     /* (l_max+1, n_max)  pi^1.5 (eta/p)^l p^-1.5        */
+    const double* gto_beta;     /* (l_max+1, n_max, n_max)  Loewdin S_l^-1/2        */
+    const double* poly_rx;      /* (quad_n)          quadrature nodes on [0, r_cut] */
+    const double* poly_rxw;     /* (quad_n)          4 pi rx^2 w_q                  */
+    const double* poly_gss;     /* (n_max, quad_n)   orthonormal g_n(rx_q)          */
+    const double* poly_B0;      /* (n_max, l_max+1)  r -> 0 limit of B              */
+    const double* poly_Bh0;     /* (n_max, l_max+1)  r -> 0 limit of Bhat           */
+    const double* self_l0;      /* (n_max)           l=0 self coefficient (incl. w0, Y00) */
+} cusoap_basis_t;
+
+/* Cell-list geometry of one neighbour search. */
+typedef struct cusoap_nl_plan {
+    double origin[3];
+    double cell[3];
+    int32_t dims[3];
+    int32_t n_cells;
+    int64_t n_ext;              /* n_images * n_atoms                                */
+    size_t ws_bytes;            /* workspace needed by nl_count / nl_fill            */
+} cusoap_nl_plan_t;
+
+/* Library information. */
+const char* cusoap_last_error(void);
+int cusoap_version(void);
+int cusoap_max_l(void);
+int cusoap_max_n(void);
+
+/* Kernel timing: when enabled, the library records HIP events around its main
+ * kernels on the launch stream ("nl_count", "nl_fill", "coefficients",
+ * "power_spectrum", "cc_projection", "jacobian_edges", "jacobian_dense",
+ * "vjp_adjoint", "vjp_edges").  cusoap_profile_get synchronises on the recorded
+ * events, returns the summed milliseconds and launch count of `name`, and
+ * clears them.  No reference counterpart: it backs create_with_profile
+ * (soap_generator.py:1815-1868) and the benchmark. */
+void cusoap_profile_enable(int32_t on);
+int cusoap_profile_get(const char* name, double* total_ms, int32_t* count);
+
+/* ---- neighbour search (replaces _build_neighbor_list / _radius_edges /
+ *      _extend_periodic, soap_generator.py:1015-1181) ---------------------
+ * Ext index of a neighbour = image * n_atoms + atom, images in the meshgrid-'ij'
+ * order of the reference's _pbc_offsets (soap_generator.py:382-396); shifts
+ * are (n_img, 3) Cartesian image translations.  Edges of a center are sorted by
+ * ext index, centers ascending: the order of the reference's `nonzero`.  The
+ * distance test is d^2 = ((-2 c.n) + |c|^2) + |n|^2 <= cutoff^2 in float64
+ * with fixed summation order and no fused multiply-add.
+ * drop_self != 0 removes edges with |r| <= 1e-12 (the create() path).
+ */
+/* Bounding box of the centers and cell grid; synchronises.  `tmp` >= 16 KiB device. */
+int cusoap_nl_plan(const double* centers, int32_t n_centers, int32_t n_atoms, int32_t n_img,
+                   double cutoff, void* tmp, cusoap_nl_plan_t* plan, void* stream);
+/* Bins the images into the cell grid and counts edges per center; writes
+ * offsets (n_centers+1, int64, exclusive scan).  Synchronises and returns the
+ * edge total and the largest per-center count. */
+int cusoap_nl_count(const cusoap_nl_plan_t* plan, const double* positions, int32_t n_atoms,
+                    const double* shifts, int32_t n_img, const double* centers, int32_t n_centers,
+                    double cutoff, int32_t drop_self, void* ws, size_t ws_bytes,
+                    int64_t* offsets, int64_t* total_host, int32_t* max_count_host, void* stream);
+/* Writes the sorted ext indices (int32) of every center's edges. */
+int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_t n_atoms,
+                   const double* shifts, int32_t n_img, const double* centers, int32_t n_centers,
+                   double cutoff, int32_t drop_self, void* ws, size_t ws_bytes,
+                   const int64_t* offsets, int32_t max_count, int32_t* nbr, void* stream);
+
+/* ---- density coefficients (replaces _coefficients_gto / _coefficients_polynomial
+ *      / _add_self_terms, soap_generator.py:1294-1435, 821-850, and the
+ *      Cacc accumulation of derivatives_analytical_ps, 2414-2423) ----------
+ * One fused pass per center over its edges: real solid harmonics, radial
+ * functions (with weighting), accumulation and the Loewdin contraction.
+ * create_path != 0: edges with |r| <= 1e-12 are skipped and self_l0 is added
+ * for centers with center_self_species >= 0 (index centers); otherwise every
+ * edge is kept (the closed-form derivative path).
+ * coeffs: (n_centers, n_species, n_max, (l_max+1)^2), float64 for PREC_F64 /
+ * PREC_MIXED, float32 for PREC_F32. */
+int cusoap_coefficients(const cusoap_basis_t* basis, int32_t precision,
+                        const double* positions, int32_t n_atoms, const int32_t* atom_species,
+                        const double* shifts, int32_t n_img,
+                        const double* centers, int32_t n_centers, const int32_t* center_self_species,
+                        const int64_t* offsets, const int32_t* nbr, int32_t create_path,
+                        void* coeffs, void* stream);
+
+/* ---- descriptor assembly (replaces _power_spectrum / _projection_cc /
+ *      _to_dscribe_cc, soap_generator.py:1599-1689) --------------------------
+ * feat_table: (n_feat) packed uint32 j | jd<<8 | n<<16 | nd<<22 | l<<28 in the
+ * DScribe order of _build_feature_slices (soap_generator.py:1215-1291). */
+int cusoap_power_spectrum(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype,
+                          int32_t n_centers, const uint32_t* feat_table, int32_t n_feat,
+                          void* out, int32_t out_dtype, void* stream);
+int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype,
+                         int32_t n_centers, void* out, int32_t out_dtype, void* stream);
+
+/* ---- dense analytical Jacobian (replaces derivatives_analytical_ps and its
+ *      Triton kernels _ps_uv_kernel / _ps_jacobian_dense_kernel /
+ *      _cc_jacobian_dense_kernel, soap_generator.py:50-220, 2270-2626, and the
+ *      autograd path derivatives_analytical, 2007-2113, for periodic cells,
+ *      weighting and attach) ----------------------------------------------------
+ * out: (n_centers, n_inc, 3, n_feat) in out_dtype.  coeffs: float64, from
+ * cusoap_coefficients on the same edges (create_path = 0 for the closed-form
+ * convention).  inv_atom: (n_atoms) int32, output row of each atom or -1.
+ * center_attach: (n_centers) int32 atom an index center moves with, or -1, or
+ * NULL (fixed centers, the reference default).  Edges must keep self edges. */
+int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges,
+                                   size_t* bytes);
+int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
+                          const double* positions, int32_t n_atoms, const int32_t* atom_species,
+                          const double* shifts, int32_t n_img,
+                          const double* centers, int32_t n_centers, const int32_t* center_attach,
+                          const int64_t* offsets, const int32_t* nbr, int64_t n_edges,
+                          const double* coeffs, const int32_t* inv_atom, int32_t n_inc,
+                          const uint32_t* feat_table, int32_t n_feat,
+                          void* ws, size_t ws_bytes, void* out, int32_t out_dtype, void* stream);
+
+/* ---- adjoint: vector-Jacobian product (no reference counterpart; the
+ *      reference contracts its dense Jacobian, soap_generator.py:2270-2626) ---
+ * grad_pos (n_atoms, 3) float64 receives sum_{c,f} grad_desc[c,f] d desc[c,f] / d x
+ * (it is zeroed first).  grad_desc: (n_centers, n_feat) in gd_dtype, or with
+ * gd_broadcast != 0 a single (n_feat) row used for every center.  coeffs as for
+ * the Jacobian.  center_attach as above. */
+int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes);
+int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode,
+               const double* positions, int32_t n_atoms, const int32_t* atom_species,
+               const double* shifts, int32_t n_img,
+               const double* centers, int32_t n_centers, const int32_t* center_attach,
+               const int64_t* offsets, const int32_t* nbr,
+               const double* coeffs, const int32_t* pair_feature, const void* grad_desc,
+               int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
+               void* ws, size_t ws_bytes, double* grad_pos, void* stream);
+
+/* ---- Loewdin orthonormalisation S^-1/2 in float64 (replaces
+ *      _lowdin_invsqrt, soap_generator.py:339-343, which calls torch.linalg.eigh
+ *      on the device) ----------------------------------------------------------
+ * S, out: (batch, n, n) row-major symmetric; cyclic Jacobi, eigenvalues clamped
+ * at eps before the inverse square root.  n <= 32. */
+int cusoap_lowdin_invsqrt_f64(const double* S, int32_t batch, int32_t n, double eps,
+                              double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUSOAP_AMD_H */

@@ -50,13 +50,29 @@ def lowdin_invsqrt(S: np.ndarray, eps: float = 1e-12) -> np.ndarray:
     return (V * (1.0 / np.sqrt(w))[None, :]) @ V.T
 
 
+def torch_linspace(start: float, end: float, n: int) -> np.ndarray:
+    """torch.linspace(start, end, n) in float64, as used at SG:694.
+
+    step = (end-start)/(n-1); entries below n//2 are fma(step, i, start), the rest
+    fma(-step, n-1-i, end), each rounded once (computed exactly with fractions).
+    Pinned against torch.linspace itself in the tests.
+    """
+    from fractions import Fraction
+    if n == 1:
+        return np.array([float(start)])
+    step = (end - start) / (n - 1)
+    return np.array([float(Fraction(start) + Fraction(step) * i) if i < n // 2
+                     else float(Fraction(end) - Fraction(step) * (n - 1 - i)) for i in range(n)])
+
+
 def gto_basis(r_cut: float, n_max: int, l_max: int) -> Tuple[np.ndarray, np.ndarray]:
     """GTO exponents alpha (l_max+1, n_max) and Loewdin betas (l_max+1, n, n).
 
-    Follows SG:690-708: a = linspace(1, r_cut, n_max),
+    Follows SG:690-708: a = torch.linspace(1, r_cut, n_max) (restated by
+    :func:`torch_linspace`),
     alpha_l = -ln(1e-3 / a^l) / a^2, S_l,ij = Gamma(l+3/2)/2 (a_i+a_j)^-(l+3/2).
     """
-    a = np.linspace(1.0, r_cut, n_max)
+    a = torch_linspace(1.0, r_cut, n_max)
     alphas, betas = [], []
     for l in range(l_max + 1):
         alpha_l = -np.log(1e-3 / (a ** l)) / (a ** 2)

@@ -474,3 +474,11 @@ def test_vjp_matches_dense_contraction():
         ref = np.einsum("cadf,cf->ad", J, w)
         F = o.vjp(s, w)
         np.testing.assert_allclose(F, ref, atol=1e-11 * np.abs(ref).max())
+
+
+def test_torch_linspace_restatement():
+    import torch
+    for n in range(1, 24):
+        for rc in (3.3, 5.0, 6.0, 7.3, 10.0, 11.7):
+            ref = torch.linspace(1.0, rc, n, dtype=torch.float64).numpy()
+            np.testing.assert_array_equal(so.torch_linspace(1.0, rc, n), ref)
