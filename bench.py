@@ -37,6 +37,8 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-centers", type=int, default=48)
     ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GB",
+                    help="only run one plain device-to-device copy of GB gigabytes (PMC counter calibration)")
     return ap.parse_args()
 
 
@@ -76,6 +78,15 @@ def main():
     from cusoap_amd import SOAP, _lib, systems
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration > 0:
+        n = int(args.copy_calibration * 1e9) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
     params = dict(r_cut=10.0, n_max=7, l_max=3, sigma=0.5, species=[1, 8], rbf="gto", average="off")
     system = systems.water_cluster_3000(seed=0)
     soap = SOAP(device=dev, dtype="float32", precision=args.precision, **params)

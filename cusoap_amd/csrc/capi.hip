@@ -56,7 +56,7 @@ int coefficients_impl(const cusoap_basis_t*, int, const double*, int, const int3
                       const double*, int, const int32_t*, const int64_t*, const int32_t*, int, void*, hipStream_t);
 int power_spectrum_impl(const cusoap_basis_t*, const void*, int, int, const uint32_t*, int, void*, int, hipStream_t);
 int cc_projection_impl(const cusoap_basis_t*, const void*, int, int, void*, int, hipStream_t);
-size_t jacobian_ws_bytes(const cusoap_basis_t*, int, int64_t);
+size_t jacobian_ws_bytes(const cusoap_basis_t*, int, int64_t, int);
 int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int,
                         const double*, int, const int32_t*, const int64_t*, const int32_t*, int64_t, const double*,
                         const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, hipStream_t);
@@ -144,9 +144,10 @@ int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_
     return cc_projection_impl(basis, coeffs, coeffs_dtype, n_centers, out, out_dtype, (hipStream_t)stream);
 }
 
-int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges, size_t* bytes) {
+int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges, int32_t n_centers,
+                                   size_t* bytes) {
     if (!basis || !bytes) return fail(CUSOAP_ERR_ARG, "jacobian_workspace_size: null argument");
-    *bytes = jacobian_ws_bytes(basis, mode, n_edges);
+    *bytes = jacobian_ws_bytes(basis, mode, n_edges, n_centers);
     return CUSOAP_OK;
 }
 

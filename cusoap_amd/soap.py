@@ -674,7 +674,8 @@ class SOAP:
         n_feat = self.get_number_of_features()
         deriv = torch.empty((x.n_centers, n_inc, 3, n_feat), dtype=self.dtype, device=self.device)
         nbytes = C.c_size_t(0)
-        _lib.check(lib.cusoap_jacobian_workspace_size(C.byref(b), mode, nl_keep.n_edges, C.byref(nbytes)),
+        _lib.check(lib.cusoap_jacobian_workspace_size(C.byref(b), mode, nl_keep.n_edges, x.n_centers,
+                                                      C.byref(nbytes)),
                    "jacobian_workspace_size")
         ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
         inv_d = self._dev(inv)

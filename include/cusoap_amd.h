@@ -159,7 +159,7 @@ int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_
  * center_attach: (n_centers) int32 atom an index center moves with, or -1, or
  * NULL (fixed centers, the reference default).  Edges must keep self edges. */
 int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges,
-                                   size_t* bytes);
+                                   int32_t n_centers, size_t* bytes);
 int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
                           const double* positions, int32_t n_atoms, const int32_t* atom_species,
                           const double* shifts, int32_t n_img,
