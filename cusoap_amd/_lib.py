@@ -49,9 +49,9 @@ _SIGS = {
     "cusoap_coefficients": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i32, p, p],
     "cusoap_power_spectrum": [C.POINTER(Basis), p, i32, i32, p, i32, p, i32, p],
     "cusoap_cc_projection": [C.POINTER(Basis), p, i32, i32, p, i32, p],
-    "cusoap_jacobian_workspace_size": [C.POINTER(Basis), i32, i64, i32, C.POINTER(sz)],
+    "cusoap_jacobian_workspace_size": [C.POINTER(Basis), i32, i64, i32, i32, C.POINTER(sz)],
     "cusoap_jacobian_dense": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i64, p, p, i32,
-                              p, i32, p, sz, p, i32, p],
+                              p, i32, p, sz, p, i32, i32, p],
     "cusoap_vjp_workspace_size": [C.POINTER(Basis), i32, C.POINTER(sz)],
     "cusoap_vjp": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, p, i32, i32, i32,
                    p, sz, p, p],

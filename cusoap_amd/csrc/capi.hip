@@ -56,10 +56,10 @@ int coefficients_impl(const cusoap_basis_t*, int, const double*, int, const int3
                       const double*, int, const int32_t*, const int64_t*, const int32_t*, int, void*, hipStream_t);
 int power_spectrum_impl(const cusoap_basis_t*, const void*, int, int, const uint32_t*, int, void*, int, hipStream_t);
 int cc_projection_impl(const cusoap_basis_t*, const void*, int, int, void*, int, hipStream_t);
-size_t jacobian_ws_bytes(const cusoap_basis_t*, int, int64_t, int);
+size_t jacobian_ws_bytes(const cusoap_basis_t*, int, int64_t, int, int);
 int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int,
                         const double*, int, const int32_t*, const int64_t*, const int32_t*, int64_t, const double*,
-                        const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, hipStream_t);
+                        const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, int, hipStream_t);
 size_t vjp_ws_bytes(const cusoap_basis_t*, int);
 int vjp_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int, const double*, int,
              const int32_t*, const int64_t*, const int32_t*, const double*, const int32_t*, const void*, int, int, int,
@@ -145,9 +145,9 @@ int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_
 }
 
 int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges, int32_t n_centers,
-                                   size_t* bytes) {
+                                   int32_t n_inc, size_t* bytes) {
     if (!basis || !bytes) return fail(CUSOAP_ERR_ARG, "jacobian_workspace_size: null argument");
-    *bytes = jacobian_ws_bytes(basis, mode, n_edges, n_centers);
+    *bytes = jacobian_ws_bytes(basis, mode, n_edges, n_centers, n_inc);
     return CUSOAP_OK;
 }
 
@@ -156,10 +156,10 @@ int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode, const doubl
                           int32_t n_centers, const int32_t* center_attach, const int64_t* offsets, const int32_t* nbr,
                           int64_t n_edges, const double* coeffs, const int32_t* inv_atom, int32_t n_inc,
                           const uint32_t* feat_table, int32_t n_feat, void* ws, size_t ws_bytes, void* out,
-                          int32_t out_dtype, void* stream) {
+                          int32_t out_dtype, int32_t compute_dtype, void* stream) {
     return jacobian_dense_impl(basis, mode, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
                                center_attach, offsets, nbr, n_edges, coeffs, inv_atom, n_inc, feat_table, n_feat, ws,
-                               ws_bytes, out, out_dtype, (hipStream_t)stream);
+                               ws_bytes, out, out_dtype, compute_dtype, (hipStream_t)stream);
 }
 
 int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes) {

@@ -25,11 +25,12 @@
 //                  contractions (thread per output) into one float64 record
 //                  per edge:  rvec[3] | Bw[n][l] | Bt[n][l] | ps: U[Sn][l], V[3][Sn][l]
 //                                                              cc: Y[L2], dY[3][L2]
-//  dense_kernel    one workgroup per (center, 64-row chunk); each wavefront
-//                  takes one output row at a time: rows without an edge are
-//                  stored as 16-byte non-temporal zeros, rows with one edge are
-//                  computed from the record staged in LDS and stored 4 elements
-//                  per lane.  Every output element is stored exactly once.
+//  dense_kernel    one workgroup per (center, 64-row chunk): each run of
+//                  consecutive rows without an edge is stored by the whole
+//                  workgroup as one flat range of 16-byte zeros; then each
+//                  wavefront takes one row with edges at a time, computed from
+//                  the record staged in LDS, 4 elements per lane.  Every output
+//                  element is stored exactly once.
 #include <cmath>
 
 #include "common.hpp"
@@ -40,6 +41,7 @@ namespace {
 
 constexpr int kJacCap = 4096;   // max included edges per center (row-key sort in LDS)
 constexpr int kRowsPerBlock = 64;
+constexpr int kChunkKeys = 512;  // keys of one 64-row chunk staged in LDS (more: read from global)
 
 struct RecLayout {
     int NL, SNL, L1, L2, stride, offB, offBt, offU, offV;
@@ -93,7 +95,8 @@ __global__ __launch_bounds__(256) void rowkey_kernel(int n_atoms, const int64_t*
                                                      const int32_t* __restrict__ nbr,
                                                      const int32_t* __restrict__ inv_atom,
                                                      unsigned long long* __restrict__ rowkeys,
-                                                     int32_t* __restrict__ kcount, int32_t* __restrict__ err) {
+                                                     int32_t* __restrict__ kcount, int chunks,
+                                                     int32_t* __restrict__ chunk_lo, int32_t* __restrict__ err) {
     __shared__ unsigned long long keys[kJacCap];
     __shared__ int nkeys;
     const int c = blockIdx.x;
@@ -110,6 +113,7 @@ __global__ __launch_bounds__(256) void rowkey_kernel(int n_atoms, const int64_t*
     __syncthreads();
     const int k = nkeys;
     if (k > kJacCap) {
+        for (int ch = threadIdx.x; ch <= chunks; ch += blockDim.x) chunk_lo[(size_t)c * (chunks + 1) + ch] = 0;
         if (threadIdx.x == 0) {
             atomicOr(err, 1);
             kcount[c] = 0;
@@ -135,6 +139,17 @@ __global__ __launch_bounds__(256) void rowkey_kernel(int n_atoms, const int64_t*
             __syncthreads();
         }
     for (int i = threadIdx.x; i < k; i += blockDim.x) rowkeys[e0 + i] = keys[i];
+    // first key of every 64-row chunk (chunk_lo[chunks] = k)
+    for (int ch = threadIdx.x; ch <= chunks; ch += blockDim.x) {
+        const int row = ch * kRowsPerBlock;
+        int lo = 0, hi = k;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)(keys[mid] >> 32) < row) lo = mid + 1;
+            else hi = mid;
+        }
+        chunk_lo[(size_t)c * (chunks + 1) + ch] = lo;
+    }
     if (threadIdx.x == 0) kcount[c] = k;
 }
 
@@ -343,14 +358,14 @@ template <>
 __device__ __forceinline__ void store4<float>(float* p, double a, double b, double c, double d) {
     typedef float f32x4 __attribute__((ext_vector_type(4)));
     const f32x4 v = {(float)a, (float)b, (float)c, (float)d};
-    __builtin_nontemporal_store(v, (f32x4*)p);
+    *(f32x4*)p = v;
 }
 template <>
 __device__ __forceinline__ void store4<double>(double* p, double a, double b, double c, double d) {
     typedef double f64x2 __attribute__((ext_vector_type(2)));
     const f64x2 v0 = {a, b}, v1 = {c, d};
-    __builtin_nontemporal_store(v0, (f64x2*)p);
-    __builtin_nontemporal_store(v1, (f64x2*)(p + 2));
+    *(f64x2*)p = v0;
+    *(f64x2*)(p + 2) = v1;
 }
 
 // lower bound of `row` in keys[lo, hi) (keys sorted; row in the upper 32 bits)
@@ -363,8 +378,50 @@ __device__ __forceinline__ int lower_row(const unsigned long long* keys, int lo,
     return lo;
 }
 
-// VEC: the row length is a multiple of 4 elements and the output is 16-byte aligned
-template <typename TO, bool VEC>
+// Feature plan of the power spectrum, per feature f (decoded once per workgroup):
+// species j, jd | l | record offsets a1 = n l, a2 = nd l (into Bw/Bt) and
+// q1 = (jd nd) l, q2 = (j n) l (into U/V).
+struct FPlan {
+    int j, jd, l, a1, a2, q1, q2;
+};
+
+__device__ __forceinline__ unsigned long long pack_plan(int j, int jd, int l, int a1, int a2, int q1, int q2) {
+    return (unsigned long long)j | ((unsigned long long)jd << 8) | ((unsigned long long)l << 16) |
+           ((unsigned long long)a1 << 20) | ((unsigned long long)a2 << 30) | ((unsigned long long)q1 << 40) |
+           ((unsigned long long)q2 << 52);
+}
+
+__device__ __forceinline__ FPlan unpack_plan(unsigned long long v) {
+    FPlan p;
+    p.j = (int)(v & 0xff);
+    p.jd = (int)((v >> 8) & 0xff);
+    p.l = (int)((v >> 16) & 0xf);
+    p.a1 = (int)((v >> 20) & 0x3ff);
+    p.a2 = (int)((v >> 30) & 0x3ff);
+    p.q1 = (int)((v >> 40) & 0xfff);
+    p.q2 = (int)((v >> 52) & 0xfff);
+    return p;
+}
+
+// one power-spectrum Jacobian element of a single edge record (compute type TC)
+template <typename TC>
+__device__ __forceinline__ TC ps_elem(const TC* rp, const RecLayout& R, int s, int d, unsigned long long plan,
+                                      const TC* prefs) {
+    const FPlan p = unpack_plan(plan);
+    const TC rv = rp[d];
+    const TC* B = rp + R.offB;
+    const TC* Bt = rp + R.offBt;
+    const TC* U = rp + R.offU;
+    const TC* V = rp + R.offV + d * R.SNL;
+    TC v = TC(0);
+    if (p.j == s) v += rv * Bt[p.a1] * U[p.q1] + B[p.a1] * V[p.q1];
+    if (p.jd == s) v += rv * Bt[p.a2] * U[p.q2] + B[p.a2] * V[p.q2];
+    return prefs[p.l] * v;
+}
+
+// VEC: the row length is a multiple of 4 elements and the output is 16-byte aligned.
+// TC: arithmetic type of the single-edge rows (records are float64 in memory).
+template <typename TO, typename TC, bool VEC>
 __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, int n_atoms,
                                                     const int32_t* __restrict__ atom_sp,
                                                     const int32_t* __restrict__ center_attach,
@@ -372,84 +429,144 @@ __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, 
                                                     const int32_t* __restrict__ nbr,
                                                     const int32_t* __restrict__ inv_atom,
                                                     const unsigned long long* __restrict__ rowkeys,
-                                                    const int32_t* __restrict__ kcount, int n_inc, int chunks,
+                                                    const int32_t* __restrict__ chunk_lo, int n_inc, int chunks,
                                                     const uint32_t* __restrict__ ft, int n_feat,
                                                     const double* __restrict__ rec, TO* __restrict__ out) {
+    __shared__ unsigned long long ckeys[kChunkKeys];
+    __shared__ int rstart[kRowsPerBlock + 1];
+    __shared__ int nzrows[kRowsPerBlock];
+    __shared__ int nnz;
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = b.n_max, S = b.n_species;
     const RecLayout R = rec_layout(S, n, b.l_max, mode);
-    double* prefs = (double*)smem;                          // [16]
-    double* recs = prefs + 16;                              // [4][stride]
-    uint32_t* fts = (uint32_t*)(recs + 4 * R.stride);       // [n_feat]
+    double* prefs = (double*)smem;                                   // [16]
+    TC* prefc = (TC*)(prefs + 16);                                   // [16]
+    unsigned long long* plan = (unsigned long long*)(prefs + 32);    // [n_feat] (ps)
+    uint32_t* fts = (uint32_t*)(plan + n_feat);                      // [n_feat] (ps)
+    TC* recs = (TC*)(fts + n_feat + (n_feat & 3));                   // [4][stride]
     const int c = blockIdx.x / chunks, ch = blockIdx.x - c * chunks;
     const int r0 = ch * kRowsPerBlock;
     const int r1 = (r0 + kRowsPerBlock < n_inc) ? r0 + kRowsPerBlock : n_inc;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t e0 = offsets[c], e1 = offsets[c + 1];
-    const int kc = kcount[c];
-    const unsigned long long* keys = rowkeys + e0;
-    if (mode == CUSOAP_MODE_PS)
-        for (int i = threadIdx.x; i < n_feat; i += blockDim.x) fts[i] = ft[i];
-    if ((int)threadIdx.x <= b.l_max) prefs[threadIdx.x] = sqrt(8.0 * M_PI * M_PI / (2 * threadIdx.x + 1));
+    const int lo = chunk_lo[(size_t)c * (chunks + 1) + ch], hi = chunk_lo[(size_t)c * (chunks + 1) + ch + 1];
+    const bool in_lds = hi - lo <= kChunkKeys;
+    const unsigned long long* keys = in_lds ? ckeys : rowkeys + e0 + lo;
+    if (in_lds)
+        for (int i = threadIdx.x; i < hi - lo; i += blockDim.x) ckeys[i] = rowkeys[e0 + lo + i];
+    if (mode == CUSOAP_MODE_PS) {
+        for (int f = threadIdx.x; f < n_feat; f += blockDim.x) {
+            const uint32_t v = ft[f];
+            fts[f] = v;
+            const int j = (int)(v & 0xffu), jd = (int)((v >> 8) & 0xffu);
+            const int nn = (int)((v >> 16) & 0x3fu), nd = (int)((v >> 22) & 0x3fu), l = (int)(v >> 28);
+            plan[f] = pack_plan(j, jd, l, nn * R.L1 + l, nd * R.L1 + l, (jd * n + nd) * R.L1 + l,
+                                (j * n + nn) * R.L1 + l);
+        }
+    }
+    if ((int)threadIdx.x <= b.l_max) {
+        const double pf = sqrt(8.0 * M_PI * M_PI / (2 * threadIdx.x + 1));
+        prefs[threadIdx.x] = pf;
+        prefc[threadIdx.x] = (TC)pf;
+    }
     int att_row = -1;
     if (center_attach) {
         const int aa = center_attach[c];
         if (aa >= 0) att_row = inv_atom[aa];
     }
     __syncthreads();
-    const int lo = lower_row(keys, 0, kc, r0);
-    const int hi = lower_row(keys, lo, kc, r1);
-    const size_t row_len = 3 * (size_t)n_feat;
-    double* myrec = recs + wv * R.stride;
-    for (int r = r0 + wv; r < r1; r += 4) {
-        TO* dst = out + ((size_t)c * n_inc + r) * row_len;
-        const int q0 = lower_row(keys, lo, hi, r);
-        const int q1 = lower_row(keys, q0, hi, r + 1);
-        if (q0 == q1 && r != att_row) {
-            if (VEC) {
-                typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-                const u32x4 z = {0u, 0u, 0u, 0u};
-                u32x4* p = (u32x4*)dst;
-                const size_t nv = row_len * sizeof(TO) / 16;
-                for (size_t i = lane; i < nv; i += 64) __builtin_nontemporal_store(z, &p[i]);
-            } else {
-                for (size_t i = lane; i < row_len; i += 64) dst[i] = TO(0);
+    for (int i = threadIdx.x; i <= r1 - r0; i += blockDim.x) rstart[i] = lower_row(keys, 0, hi - lo, r0 + i);
+    __syncthreads();
+    const int row_len = 3 * n_feat;
+    // 1) every run of consecutive rows without an edge: one flat block-wide store
+    if (threadIdx.x == 0) nnz = 0;
+    {
+        int r = r0;
+        while (r < r1) {
+            while (r < r1 && (rstart[r - r0] != rstart[r - r0 + 1] || r == att_row)) ++r;
+            const int rs = r;
+            while (r < r1 && rstart[r - r0] == rstart[r - r0 + 1] && r != att_row) ++r;
+            if (r > rs) {
+                TO* dst = out + ((size_t)c * n_inc + rs) * (size_t)row_len;
+                const size_t cnt = (size_t)(r - rs) * row_len;
+                if (VEC) {
+                    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+                    const u32x4 z = {0u, 0u, 0u, 0u};
+                    u32x4* p = (u32x4*)dst;
+                    const size_t nv = cnt * sizeof(TO) / 16;
+                    for (size_t i = threadIdx.x; i < nv; i += blockDim.x) p[i] = z;
+                } else {
+                    for (size_t i = threadIdx.x; i < cnt; i += blockDim.x) dst[i] = TO(0);
+                }
             }
-            continue;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)(r1 - r0)) {
+        const int r = r0 + threadIdx.x;
+        if (rstart[r - r0] != rstart[r - r0 + 1] || r == att_row) nzrows[atomicAdd(&nnz, 1)] = r;
+    }
+    __syncthreads();
+    // 2) rows with edges: one wavefront per row
+    TC* myrec = recs + wv * R.stride;
+    for (int i = wv; i < nnz; i += 4) {
+        const int r = nzrows[i];
+        TO* dst = out + ((size_t)c * n_inc + r) * (size_t)row_len;
+        const int q0 = rstart[r - r0];
+        const int q1 = rstart[r - r0 + 1];
         if (q1 - q0 == 1 && r != att_row) {
             // one edge: stage its record in LDS (wavefront-private slot)
             const int64_t e = e0 + (int64_t)(keys[q0] & 0xffffffffull);
             const double* rp = rec + (size_t)e * R.stride;
-            for (int i = lane; i < R.stride; i += 64) myrec[i] = rp[i];
+            for (int k = lane; k < R.stride; k += 64) myrec[k] = (TC)rp[k];
             wave_sync();
             const int s = atom_sp[nbr[e] % n_atoms];
-            if (VEC) {
-                for (size_t t = 4 * (size_t)lane; t < row_len; t += 256) {
-                    const double v0 = elem_value(myrec, R, mode, n, s, (int)t, n_feat, fts, prefs);
-                    const double v1 = elem_value(myrec, R, mode, n, s, (int)t + 1, n_feat, fts, prefs);
-                    const double v2 = elem_value(myrec, R, mode, n, s, (int)t + 2, n_feat, fts, prefs);
-                    const double v3 = elem_value(myrec, R, mode, n, s, (int)t + 3, n_feat, fts, prefs);
-                    store4<TO>(dst + t, v0, v1, v2, v3);
+            if (mode == CUSOAP_MODE_PS && VEC) {
+                int t = 4 * lane;
+                int d = t / n_feat, f = t - d * n_feat;
+                for (; t < row_len; t += 256) {
+                    TC v[4];
+                    int dd = d, ff = f;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (ff >= n_feat) {
+                            ff -= n_feat;
+                            ++dd;
+                        }
+                        v[k] = ps_elem<TC>(myrec, R, s, dd, plan[ff], prefc);
+                        ++ff;
+                    }
+                    store4<TO>(dst + t, (double)v[0], (double)v[1], (double)v[2], (double)v[3]);
+                    f += 256;
+                    while (f >= n_feat) {
+                        f -= n_feat;
+                        ++d;
+                    }
+                }
+            } else if (mode == CUSOAP_MODE_PS) {
+                for (int t = lane; t < row_len; t += 64) {
+                    const int d = t / n_feat, f = t - d * n_feat;
+                    dst[t] = (TO)ps_elem<TC>(myrec, R, s, d, plan[f], prefc);
                 }
             } else {
-                for (size_t t = lane; t < row_len; t += 64)
-                    dst[t] = (TO)elem_value(myrec, R, mode, n, s, (int)t, n_feat, fts, prefs);
+                const double* rpd = rec + (size_t)e * R.stride;
+                for (int t = lane; t < row_len; t += 64)
+                    dst[t] = (TO)elem_value(rpd, R, mode, n, s, t, n_feat, fts, prefs);
             }
             wave_sync();
             continue;
         }
-        // several edges (periodic images) and/or the attach row: read records from global
-        for (size_t t = lane; t < row_len; t += 64) {
+        // several edges (periodic images) and/or the attach row: float64 from the global records
+        for (int t = lane; t < row_len; t += 64) {
             double v = 0.0;
             for (int q = q0; q < q1; ++q) {
                 const int64_t e = e0 + (int64_t)(keys[q] & 0xffffffffull);
-                v += elem_value(rec + (size_t)e * R.stride, R, mode, n, atom_sp[nbr[e] % n_atoms], (int)t, n_feat,
+                v += elem_value(rec + (size_t)e * R.stride, R, mode, n, atom_sp[nbr[e] % n_atoms], t, n_feat,
                                 fts, prefs);
             }
             if (r == att_row)
                 for (int64_t e = e0; e < e1; ++e)
-                    v -= elem_value(rec + (size_t)e * R.stride, R, mode, n, atom_sp[nbr[e] % n_atoms], (int)t,
+                    v -= elem_value(rec + (size_t)e * R.stride, R, mode, n, atom_sp[nbr[e] % n_atoms], t,
                                     n_feat, fts, prefs);
             dst[t] = (TO)v;
         }
@@ -492,23 +609,25 @@ int launch_records(const cusoap_basis_t& b, int mode, const double* pos, int n_a
     return CUSOAP_OK;
 }
 
-template <typename TO, bool VEC>
+template <typename TO, typename TC, bool VEC>
 int launch_dense(const cusoap_basis_t& b, int mode, int n_atoms, const int32_t* atom_sp, const int32_t* center_attach,
                  const int64_t* offsets, const int32_t* nbr, const int32_t* inv_atom,
-                 const unsigned long long* rowkeys, const int32_t* kcount, int n_centers, int n_inc,
+                 const unsigned long long* rowkeys, const int32_t* chunk_lo, int n_centers, int n_inc,
                  const uint32_t* ft, int n_feat, const double* rec, void* out, hipStream_t st) {
     const RecLayout R = rec_layout(b.n_species, b.n_max, b.l_max, mode);
-    const size_t smem = sizeof(double) * (16 + 4 * (size_t)R.stride) + sizeof(uint32_t) * (size_t)n_feat;
+    if (R.SNL >= 4096 || R.NL >= 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: too many radial channels");
+    const size_t smem = sizeof(double) * 32 + (sizeof(unsigned long long) + sizeof(uint32_t)) * (size_t)n_feat +
+                        sizeof(uint32_t) * 4 + sizeof(TC) * 4 * (size_t)R.stride;
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: record too large for LDS");
     const int chunks = (n_inc + kRowsPerBlock - 1) / kRowsPerBlock;
     const int64_t grid = (int64_t)n_centers * chunks;
     if (grid >= ((int64_t)1 << 31)) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: output too large");
-    auto kern = dense_kernel<TO, VEC>;
+    auto kern = dense_kernel<TO, TC, VEC>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("jacobian_dense", st);
     kern<<<(unsigned)grid, 256, smem, st>>>(b, mode, n_atoms, atom_sp, center_attach, offsets, nbr, inv_atom, rowkeys,
-                                            kcount, n_inc, chunks, ft, n_feat, rec, (TO*)out);
+                                            chunk_lo, n_inc, chunks, ft, n_feat, rec, (TO*)out);
     prof_end("jacobian_dense", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -518,11 +637,12 @@ struct JacWs {
     double* rec;
     unsigned long long* rowkeys;
     int32_t* kcount;
+    int32_t* chunk_lo;
     int32_t* err;
     size_t bytes;
 };
 
-JacWs carve_jac(void* base, const RecLayout& R, int64_t n_edges, int n_centers) {
+JacWs carve_jac(void* base, const RecLayout& R, int64_t n_edges, int n_centers, int n_inc) {
     JacWs w;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -534,6 +654,8 @@ JacWs carve_jac(void* base, const RecLayout& R, int64_t n_edges, int n_centers) 
     w.rec = (double*)take(sizeof(double) * (size_t)R.stride * (size_t)E);
     w.rowkeys = (unsigned long long*)take(sizeof(unsigned long long) * (size_t)E);
     w.kcount = (int32_t*)take(sizeof(int32_t) * (size_t)std::max(n_centers, 1));
+    const size_t chunks = (size_t)(std::max(n_inc, 1) + kRowsPerBlock - 1) / kRowsPerBlock;
+    w.chunk_lo = (int32_t*)take(sizeof(int32_t) * (size_t)std::max(n_centers, 1) * (chunks + 1));
     w.err = (int32_t*)take(sizeof(int32_t) * 4);
     w.bytes = off;
     return w;
@@ -541,27 +663,29 @@ JacWs carve_jac(void* base, const RecLayout& R, int64_t n_edges, int n_centers) 
 
 }  // namespace
 
-size_t jacobian_ws_bytes(const cusoap_basis_t* b, int mode, int64_t n_edges, int n_centers) {
+size_t jacobian_ws_bytes(const cusoap_basis_t* b, int mode, int64_t n_edges, int n_centers, int n_inc) {
     const RecLayout R = rec_layout(b->n_species, b->n_max, b->l_max, mode);
-    return carve_jac(nullptr, R, n_edges, n_centers).bytes;
+    return carve_jac(nullptr, R, n_edges, n_centers, n_inc).bytes;
 }
 
 int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, int n_atoms, const int32_t* atom_sp,
                         const double* shifts, int n_img, const double* centers, int n_centers,
                         const int32_t* center_attach, const int64_t* offsets, const int32_t* nbr, int64_t n_edges,
                         const double* coeffs, const int32_t* inv_atom, int n_inc, const uint32_t* ft, int n_feat,
-                        void* ws, size_t ws_bytes, void* out, int odt, hipStream_t st) {
+                        void* ws, size_t ws_bytes, void* out, int odt, int cdt, hipStream_t st) {
     (void)n_img;
     if (!b || b->n_max < 1 || b->n_max > kMaxN || b->l_max < 0 || b->l_max > kMaxL)
         return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: n_max must be 1..16 and l_max 0..9");
     if (mode != CUSOAP_MODE_PS && mode != CUSOAP_MODE_CC) return fail(CUSOAP_ERR_ARG, "jacobian: unknown mode");
     if (b->n_species > 255 || b->n_max > 63) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: feature table range");
     const RecLayout R = rec_layout(b->n_species, b->n_max, b->l_max, mode);
-    JacWs w = carve_jac(ws, R, n_edges, n_centers);
+    JacWs w = carve_jac(ws, R, n_edges, n_centers, n_inc);
     if (ws_bytes < w.bytes) return fail(CUSOAP_ERR_WORKSPACE, "jacobian: workspace too small");
     if (n_centers == 0 || n_inc == 0 || n_feat == 0) return CUSOAP_OK;
     CUSOAP_HIP(hipMemsetAsync(w.err, 0, sizeof(int32_t), st));
-    rowkey_kernel<<<n_centers, 256, 0, st>>>(n_atoms, offsets, nbr, inv_atom, w.rowkeys, w.kcount, w.err);
+    const int chunks = (n_inc + kRowsPerBlock - 1) / kRowsPerBlock;
+    rowkey_kernel<<<n_centers, 256, 0, st>>>(n_atoms, offsets, nbr, inv_atom, w.rowkeys, w.kcount, chunks,
+                                             w.chunk_lo, w.err);
     CUSOAP_LAUNCH_CHECK();
     int rc = CUSOAP_OK;
 #define CUSOAP_CASE(L) \
@@ -575,11 +699,12 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
     if (rc) return rc;
     // 4-element groups per lane: every row length must be a multiple of 4 elements
     const bool vec = ((3 * (size_t)n_feat) % 4 == 0) && (((uintptr_t)out & 15) == 0);
-#define CUSOAP_DENSE(TO, V) \
-    launch_dense<TO, V>(*b, mode, n_atoms, atom_sp, center_attach, offsets, nbr, inv_atom, w.rowkeys, w.kcount, \
-                        n_centers, n_inc, ft, n_feat, w.rec, out, st)
-    if (odt == CUSOAP_DT_F32) rc = vec ? CUSOAP_DENSE(float, true) : CUSOAP_DENSE(float, false);
-    else rc = vec ? CUSOAP_DENSE(double, true) : CUSOAP_DENSE(double, false);
+#define CUSOAP_DENSE(TO, TC, V) \
+    launch_dense<TO, TC, V>(*b, mode, n_atoms, atom_sp, center_attach, offsets, nbr, inv_atom, w.rowkeys, w.chunk_lo, \
+                            n_centers, n_inc, ft, n_feat, w.rec, out, st)
+    if (odt == CUSOAP_DT_F32 && cdt == CUSOAP_DT_F32) rc = vec ? CUSOAP_DENSE(float, float, true) : CUSOAP_DENSE(float, float, false);
+    else if (odt == CUSOAP_DT_F32) rc = vec ? CUSOAP_DENSE(float, double, true) : CUSOAP_DENSE(float, double, false);
+    else rc = vec ? CUSOAP_DENSE(double, double, true) : CUSOAP_DENSE(double, double, false);
 #undef CUSOAP_DENSE
     if (rc) return rc;
     int32_t herr = 0;

@@ -675,19 +675,20 @@ class SOAP:
         deriv = torch.empty((x.n_centers, n_inc, 3, n_feat), dtype=self.dtype, device=self.device)
         nbytes = C.c_size_t(0)
         _lib.check(lib.cusoap_jacobian_workspace_size(C.byref(b), mode, nl_keep.n_edges, x.n_centers,
-                                                      C.byref(nbytes)),
+                                                      n_inc, C.byref(nbytes)),
                    "jacobian_workspace_size")
         ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
         inv_d = self._dev(inv)
         att = self._attach_array(x, centers, attach)
         odt = _lib.DT_F64 if self.dtype == torch.float64 else _lib.DT_F32
+        cdt = _lib.DT_F64 if (self.precision == "float64" or odt == _lib.DT_F64) else _lib.DT_F32
         if deriv.numel() > 0:
             _lib.check(lib.cusoap_jacobian_dense(
                 C.byref(b), mode, x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(), x.shifts.data_ptr(), x.n_img,
                 x.centers.data_ptr(), x.n_centers, att.data_ptr() if att is not None else None,
                 nl_keep.offsets.data_ptr(), nl_keep.nbr.data_ptr(), nl_keep.n_edges, coeffs.data_ptr(),
                 inv_d.data_ptr(), n_inc, self._keep["ft"].data_ptr(), n_feat, ws.data_ptr(), ws.numel(),
-                deriv.data_ptr(), odt, self._stream()), "jacobian_dense")
+                deriv.data_ptr(), odt, cdt, self._stream()), "jacobian_dense")
         if return_descriptor:
             return deriv, self._descriptor(coeffs)
         return deriv

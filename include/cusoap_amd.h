@@ -157,9 +157,12 @@ int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_
  * cusoap_coefficients on the same edges (create_path = 0 for the closed-form
  * convention).  inv_atom: (n_atoms) int32, output row of each atom or -1.
  * center_attach: (n_centers) int32 atom an index center moves with, or -1, or
- * NULL (fixed centers, the reference default).  Edges must keep self edges. */
+ * NULL (fixed centers, the reference default).  Edges must keep self edges.
+ * compute_dtype: arithmetic of rows fed by a single edge (CUSOAP_DT_F64 as the
+ * reference; CUSOAP_DT_F32 only with float32 output); records, rows with
+ * several edges and the attach row are always float64. */
 int cusoap_jacobian_workspace_size(const cusoap_basis_t* basis, int32_t mode, int64_t n_edges,
-                                   int32_t n_centers, size_t* bytes);
+                                   int32_t n_centers, int32_t n_inc, size_t* bytes);
 int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
                           const double* positions, int32_t n_atoms, const int32_t* atom_species,
                           const double* shifts, int32_t n_img,
@@ -167,7 +170,8 @@ int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
                           const int64_t* offsets, const int32_t* nbr, int64_t n_edges,
                           const double* coeffs, const int32_t* inv_atom, int32_t n_inc,
                           const uint32_t* feat_table, int32_t n_feat,
-                          void* ws, size_t ws_bytes, void* out, int32_t out_dtype, void* stream);
+                          void* ws, size_t ws_bytes, void* out, int32_t out_dtype,
+                          int32_t compute_dtype, void* stream);
 
 /* ---- adjoint: vector-Jacobian product (no reference counterpart; the
  *      reference contracts its dense Jacobian, soap_generator.py:2270-2626) ---
