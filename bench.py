@@ -8,7 +8,14 @@ create() of every center plus one derivatives(method="analytical") call, i.e.
 the full dense (3000, 3000, 3, 420) Jacobian, as the reference's serial driver
 does.  Positions are already resident in HBM when the timed region starts.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Other workloads (--config), each step = create() + the adjoint forces of a
+seeded Gaussian linear readout (descriptor_and_vjp; no dp/dR is formed):
+  water_box  periodic bulk water, 32768 H2O = 98,304 atoms at 1.0 g/cm^3 (L=99.338 A),
+             r_cut=6, sigma=0.5, n_max=8, l_max=6
+  hcno_box   periodic H/C/N/O box, 20,000 atoms at 0.10 A^-3, r_cut=5, sigma=0.4, n_max=8, l_max=6
+  h2o_1m     non-periodic (H2O)_333334 cluster, 1,000,002 atoms, r_cut=6, sigma=0.5, n_max=7, l_max=3
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--precision P] [--no-cpu-baseline]
 Prints one JSON line.
 """
 
@@ -34,6 +41,8 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="float64", choices=["float64", "mixed", "float32"])
+    ap.add_argument("--config", default="h2o_1000", choices=["h2o_1000", "water_box", "hcno_box", "h2o_1m"])
+    ap.add_argument("--compare", default="", help="comma list of precisions to compare against float64 (large configs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-centers", type=int, default=48)
     ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
@@ -87,6 +96,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
         return
+    if args.config != "h2o_1000":
+        return bench_adjoint(args, dev)
     params = dict(r_cut=10.0, n_max=7, l_max=3, sigma=0.5, species=[1, 8], rbf="gto", average="off")
     system = systems.water_cluster_3000(seed=0)
     soap = SOAP(device=dev, dtype="float32", precision=args.precision, **params)
@@ -155,6 +166,96 @@ def main():
         out["max_col_rel_err_vs_oracle"] = float((np.abs(d - ref).max(0) / np.where(sc > 0, sc, 1)).max())
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(system, params, args.cpu_centers)
+    print(json.dumps(out), flush=True)
+
+
+ADJ_CONFIGS = {
+    "water_box": ("periodic bulk water, 32768 H2O = 98,304 atoms, 1.0 g/cm^3, L=99.338 A, H/O, GTO, "
+                  "r_cut=6.0, sigma=0.5, n_max=8, l_max=6, average=off; create + adjoint forces of w.p",
+                  dict(r_cut=6.0, n_max=8, l_max=6, sigma=0.5, species=[1, 8], periodic=True)),
+    "hcno_box": ("periodic H/C/N/O box, 20,000 atoms (0.50/0.25/0.10/0.15) at 0.10 A^-3, d_min 0.9 A, GTO, "
+                 "r_cut=5.0, sigma=0.4, n_max=8, l_max=6; create + adjoint forces of w.p",
+                 dict(r_cut=5.0, n_max=8, l_max=6, sigma=0.4, species=[1, 6, 7, 8], periodic=True)),
+    "h2o_1m": ("non-periodic (H2O)_333334 cluster, 1,000,002 atoms, H/O, GTO, r_cut=6.0, sigma=0.5, n_max=7, "
+               "l_max=3, average=off; create + adjoint forces of w.p",
+               dict(r_cut=6.0, n_max=7, l_max=3, sigma=0.5, species=[1, 8])),
+}
+
+
+def make_system(name):
+    from cusoap_amd import systems
+    if name == "water_box":
+        return systems.water_box(32, seed=0)
+    if name == "hcno_box":
+        return systems.random_box(n_atoms=20000, density=0.10, seed=0)
+    return systems.water_cluster_1m(seed=0)
+
+
+def bench_adjoint(args, dev):
+    """create + adjoint forces on one of the large workloads."""
+    import torch
+    from cusoap_amd import SOAP, _lib
+    desc_txt, params = ADJ_CONFIGS[args.config]
+    t0 = time.perf_counter()
+    system = make_system(args.config)
+    t_gen = time.perf_counter() - t0
+    n_env = system["positions"].shape[0]
+    sys_dev = dict(system, positions=torch.tensor(system["positions"], device=dev))
+    soaps = {}
+
+    def soap_for(prec):
+        if prec not in soaps:
+            soaps[prec] = SOAP(device=dev, dtype="float32", precision=prec, **params)
+        return soaps[prec]
+
+    soap = soap_for(args.precision)
+    n_feat = soap.get_number_of_features()
+    w = torch.tensor(np.random.default_rng(12345).normal(size=n_feat), device=dev)
+
+    def step(sp):
+        return sp.descriptor_and_vjp(sys_dev, w)
+
+    for _ in range(args.warmup):
+        d, F = step(soap)
+        del d, F
+    torch.cuda.synchronize()
+    names = ("coefficients", "vjp_edges", "vjp_adjoint", "nl_count", "nl_fill", "power_spectrum")
+    _lib.profile_enable(True)
+    for nm in names:
+        _lib.profile_get(nm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d, F = step(soap)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _lib.profile_enable(False)
+    kern = {}
+    for nm in names:
+        ms, n = _lib.profile_get(nm)
+        kern[nm] = ms / max(n, 1)
+    out = {"metric": METRIC, "value": round(n_env * args.steps / dt, 1), "unit": "atom-envs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": {"float64": "f64", "mixed": "f64+f32", "float32": "f32"}[args.precision],
+           "data": "synthetic",
+           "config": {"workload": desc_txt, "n_atoms": n_env, "n_features": n_feat, "precision": args.precision,
+                      "system_generation_s": round(t_gen, 1), "parallelism": "single GPU"},
+           "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
+    if args.compare:
+        ref_d = d.double()
+        ref_F = F
+        if args.precision != "float64":
+            ref_d, ref_F = step(soap_for("float64"))
+            ref_d = ref_d.double()
+        errs = {}
+        for prec in [p for p in args.compare.split(",") if p]:
+            dd, FF = step(soap_for(prec))
+            dd = dd.double()
+            sc = ref_d.abs().amax(0).clamp_min(1e-300)
+            errs[prec] = {"desc_max_col_rel": float(((dd - ref_d).abs().amax(0) / sc).max()),
+                          "force_max_rel": float((FF - ref_F).abs().max() / ref_F.abs().max())}
+        out["errors_vs_gpu_float64"] = errs
     print(json.dumps(out), flush=True)
 
 

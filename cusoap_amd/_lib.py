@@ -53,7 +53,7 @@ _SIGS = {
     "cusoap_jacobian_dense": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i64, p, p, i32,
                               p, i32, p, sz, p, i32, i32, p],
     "cusoap_vjp_workspace_size": [C.POINTER(Basis), i32, C.POINTER(sz)],
-    "cusoap_vjp": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, p, i32, i32, i32,
+    "cusoap_vjp": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, i32, p, p, i32, i32, i32,
                    p, sz, p, p],
     "cusoap_lowdin_invsqrt_f64": [p, i32, i32, f64, p, p],
     "cusoap_profile_get": [C.c_char_p, C.POINTER(f64), C.POINTER(i32)],

@@ -4,18 +4,22 @@
 // is infeasible beyond ~10^4 atoms.
 //
 //   L = sum_{c,f} G_desc[c][f] desc[c][f]
-//   dL/dc[c][s][n][lm]:  power spectrum  pref_l sum_{s'n'} W_l[(s n),(s' n')] c[s'][n'][lm]
-//                        (W from G_desc through the pair -> feature table, x2 on the diagonal)
-//                        cc              G_desc permuted back to (y, z, x) and scaled
-//   dL/dx_atom = sum over edges (c -> atom) of sum_{n,lm} dL/dc[c][s][n][lm] g[d][n][lm]
-//   with the edge gradient g of the closed-form Jacobian (fixed centers).
+//   G[c][s][n][lm] = dL/dc:  power spectrum  pref_l sum_{s'n'} W_l[(s n),(s' n')] c[s'][n'][lm]
+//                            (W from G_desc through the pair -> feature table, x2 on the diagonal)
+//                            cc              G_desc permuted back to (y, z, x) and scaled
+//   dL/dx_atom = sum over edges (c -> atom) of sum_{n,lm} G[c][s][n][lm] g[d][n][lm],
+//   g the edge gradient of the closed-form Jacobian (fixed centers):
+//   g[d][n][lm] = r_d Bt_nl Y_lm + Bw_nl dY_lm/dr_d.
 //
-// Kernel 1 (one workgroup per center) forms dL/dc and, for the GTO basis,
-// folds the Loewdin matrix into it: M[s][k][lm] = sum_n beta_l[n][k] dL/dc[s][n][lm],
-// so kernel 2 contracts with the primitive radial functions directly.
-// Kernel 2 (one wavefront per center, tiles of 64 edges, lane = edge)
-// recomputes each edge's solid harmonics (with gradients) and radial factors,
-// contracts them with M and scatters the 3-vector with float64 atomics.
+// Kernel 1 (one workgroup per center) forms G and stores M[c][s][lm][k]: for
+// F64/F32 with the Loewdin matrix folded in, M = sum_n beta_l[n][k] G[s][n][lm],
+// so kernel 2 contracts with the primitive radial functions directly; for
+// MIXED, M = G and kernel 2 applies beta per edge in float64.
+// Kernel 2 (GTO; 256 threads per center, thread per edge) walks the solid
+// harmonics l-major, builds the radial vectors of degree l once per degree in
+// per-thread LDS columns and contracts: H0 = sum_k rad0_k M[s][lm][k], H1 = sum_k rad1_k M[s][lm][k],
+// grad_d += r_d Y_lm H1 + dY_lm/dr_d H0; then scatters the 3-vector with
+// float64 atomics.  The polynomial basis uses a float64 tile kernel.
 #include <cmath>
 
 #include "common.hpp"
@@ -39,21 +43,22 @@ __device__ __forceinline__ int cc_slot_of(int lm) {
     return lm;
 }
 
-template <typename TG>
-__global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, int mode, const double* __restrict__ coeffs,
+// M[c][s][lm][k] (TM): dL/dc, with beta folded in when fold_beta
+template <typename TC, typename TG, typename TM>
+__global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, int mode, const TC* __restrict__ coeffs,
                                                             const int32_t* __restrict__ pair_feature,
                                                             const TG* __restrict__ gd, int gd_broadcast, int n_feat,
-                                                            double* __restrict__ M) {
+                                                            int fold_beta, TM* __restrict__ M) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int S = b.n_species, n = b.n_max, L1 = b.l_max + 1, L2 = L1 * L1;
     const int per = S * n * L2, SN = S * n;
-    double* cs = (double*)smem;       // [per]
-    double* G = cs + per;              // [per]
+    double* cs = (double*)smem;       // [per]   c[s][n][lm]
+    double* G = cs + per;              // [per]   G[s][n][lm]
     double* g = G + per;               // [n_feat]
     const int c = blockIdx.x;
-    const double* src = coeffs + (size_t)c * per;
+    const TC* src = coeffs + (size_t)c * per;
     const TG* gsrc = gd + (gd_broadcast ? 0 : (size_t)c * n_feat);
-    for (int i = threadIdx.x; i < per; i += blockDim.x) cs[i] = src[i];
+    for (int i = threadIdx.x; i < per; i += blockDim.x) cs[i] = (double)src[i];
     for (int i = threadIdx.x; i < n_feat; i += blockDim.x) g[i] = (double)gsrc[i];
     __syncthreads();
     for (int t = threadIdx.x; t < per; t += blockDim.x) {
@@ -73,22 +78,151 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
         G[t] = v;
     }
     __syncthreads();
-    double* Mc = M + (size_t)c * per;
+    TM* Mc = M + (size_t)c * per;
     for (int t = threadIdx.x; t < per; t += blockDim.x) {
-        if (b.rbf == CUSOAP_RBF_GTO) {
-            const int a = t / L2, lm = t - a * L2, l = lm_l2(lm);
-            const int s = a / n, k = a - s * n;
-            double v = 0.0;
+        // output order [s][lm][k]
+        const int s = t / (L2 * n), rem = t - s * L2 * n;
+        const int lm = rem / n, k = rem - lm * n;
+        double v;
+        if (fold_beta) {
+            const int l = lm_l2(lm);
+            v = 0.0;
             for (int nn = 0; nn < n; ++nn) v += b.gto_beta[((size_t)l * n + nn) * n + k] * G[(s * n + nn) * L2 + lm];
-            Mc[t] = v;
         } else {
-            Mc[t] = G[t];
+            v = G[(s * n + k) * L2 + lm];
+        }
+        Mc[t] = (TM)v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T t_exp2(T v);
+template <>
+__device__ __forceinline__ float t_exp2<float>(float v) { return expf(v); }
+template <>
+__device__ __forceinline__ double t_exp2<double>(double v) { return exp(v); }
+
+// GTO adjoint, thread per edge.  TM: type of M and of the contraction.
+// MIXED: M = G (no beta), radial B/Bt per edge in float64 then rounded to TM.
+// The radial vectors of the current degree live in LDS columns [k][thread].
+template <int LMAX, typename TM, bool MIXED>
+__global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
+                                                      const int32_t* __restrict__ atom_sp,
+                                                      const double* __restrict__ shifts,
+                                                      const double* __restrict__ centers,
+                                                      const int32_t* __restrict__ center_attach,
+                                                      const int64_t* __restrict__ offsets,
+                                                      const int32_t* __restrict__ nbr, const TM* __restrict__ M,
+                                                      double* __restrict__ grad) {
+    constexpr int L1 = LMAX + 1, L2 = L1 * L1;
+    constexpr int NT = 256;
+    const int n = b.n_max, S = b.n_species;
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* qd = (double*)smem;                             // [n][NT]   (MIXED) primitives
+    TM* rad0 = (TM*)(qd + (MIXED ? n * NT : 0));            // [n][NT]
+    TM* rad1 = rad0 + n * NT;                               // [n][NT]
+    TM* Ms = rad1 + n * NT;                                 // [S][L2][n]
+    __shared__ double red[3][4];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const int per = S * n * L2;
+    for (int i = tid; i < per; i += NT) Ms[i] = M[(size_t)c * per + i];
+    __syncthreads();
+    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
+    const int64_t e0 = offsets[c], e1 = offsets[c + 1];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    TM* r0c = rad0 + tid;
+    TM* r1c = rad1 + tid;
+    double* qc = qd + tid;
+    for (int64_t e = e0 + tid; e < e1; e += NT) {
+        const int ext = nbr[e];
+        const int img = ext / n_atoms, a = ext - img * n_atoms;
+        const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
+        const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
+        const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        const double r = sqrt(r2);
+        double w = 1.0, dw = 0.0;
+        if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
+        const double dwr = r > 0.0 ? dw / r : 0.0;
+        const TM* Mrow = Ms + (size_t)atom_sp[a] * L2 * n;
+        const TM trx = (TM)rx, tryy = (TM)ry, trz = (TM)rz;
+        TM gx = TM(0), gy = TM(0), gz = TM(0);
+        auto on_l = [&](int l) {
+            if (MIXED) {
+                for (int k = 0; k < n; ++k) qc[k * NT] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);
+                for (int nn = 0; nn < n; ++nn) {
+                    const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
+                    double B = 0.0, Bh = 0.0;
+                    for (int k = 0; k < n; ++k) {
+                        const double q = qc[k * NT];
+                        B += beta[k] * q;
+                        Bh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q;
+                    }
+                    r0c[nn * NT] = (TM)(w * B);
+                    r1c[nn * NT] = (TM)(w * Bh + dwr * B);
+                }
+            } else {
+                for (int k = 0; k < n; ++k) {
+                    const TM gam = (TM)b.gto_gamma[l * n + k];
+                    const TM q = (TM)b.gto_pref[l * n + k] * t_exp2<TM>(-gam * (TM)r2);
+                    r0c[k * NT] = (TM)w * q;
+                    r1c[k * NT] = ((TM)w * (TM(-2) * gam) + (TM)dwr) * q;
+                }
+            }
+        };
+        auto put = [&](int, int lm, TM y, TM yx, TM yy, TM yz) {
+            const TM* mv = Mrow + lm * n;
+            TM h0 = TM(0), h1 = TM(0);
+            for (int k = 0; k < n; ++k) {
+                const TM m = mv[k];
+                h0 += r0c[k * NT] * m;
+                h1 += r1c[k * NT] * m;
+            }
+            const TM yh1 = y * h1;
+            gx += trx * yh1 + yx * h0;
+            gy += tryy * yh1 + yy * h0;
+            gz += trz * yh1 + yz * h0;
+        };
+        solid_harmonics_lmajor<LMAX, true, TM>(trx, tryy, trz, on_l, put);
+        atomicAdd(&grad[3 * (size_t)a], (double)gx);
+        atomicAdd(&grad[3 * (size_t)a + 1], (double)gy);
+        atomicAdd(&grad[3 * (size_t)a + 2], (double)gz);
+        sx += (double)gx;
+        sy += (double)gy;
+        sz += (double)gz;
+    }
+    if (center_attach) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+            sz += __shfl_xor(sz, o, 64);
+        }
+        const int lane = tid & 63, wv = tid >> 6;
+        if (lane == 0) {
+            red[0][wv] = sx;
+            red[1][wv] = sy;
+            red[2][wv] = sz;
+        }
+        __syncthreads();
+        const int ca = center_attach[c];
+        if (tid == 0 && ca >= 0) {
+            double t0 = 0, t1 = 0, t2 = 0;
+            for (int w2 = 0; w2 < 4; ++w2) {
+                t0 += red[0][w2];
+                t1 += red[1][w2];
+                t2 += red[2][w2];
+            }
+            atomicAdd(&grad[3 * (size_t)ca], -t0);
+            atomicAdd(&grad[3 * (size_t)ca + 1], -t1);
+            atomicAdd(&grad[3 * (size_t)ca + 2], -t2);
         }
     }
 }
 
-template <int LMAX, int RBF>
-__global__ __launch_bounds__(64) void vjp_edge_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
+// polynomial basis, float64: one wavefront per center, tiles of 64 edges, lane = edge
+template <int LMAX>
+__global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
                                                       const int32_t* __restrict__ atom_sp,
                                                       const double* __restrict__ shifts,
                                                       const double* __restrict__ centers,
@@ -99,7 +233,7 @@ __global__ __launch_bounds__(64) void vjp_edge_kernel(cusoap_basis_t b, const do
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     const int n = b.n_max, S = b.n_species, NL = n * L1;
     extern __shared__ __align__(16) unsigned char smem[];
-    double* Ms = (double*)smem;               // [S][n][L2]
+    double* Ms = (double*)smem;               // [S][L2][n]
     double* R0 = Ms + S * n * L2;             // [64][NL]
     double* R1 = R0 + 64 * NL;                // [64][NL]
     const int c = blockIdx.x, lane = threadIdx.x;
@@ -125,56 +259,45 @@ __global__ __launch_bounds__(64) void vjp_edge_kernel(cusoap_basis_t b, const do
             if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
             const double dwr = r > 0.0 ? dw / r : 0.0;
             const int s = atom_sp[a];
-            if (RBF == CUSOAP_RBF_GTO) {
-                for (int k = 0; k < n; ++k)
+            for (int k = 0; k < n; ++k) {
+                double B[L1], Bh[L1];
+                if (r < 1e-4) {
 #pragma unroll
                     for (int l = 0; l < L1; ++l) {
-                        const double gam = b.gto_gamma[l * n + k];
-                        const double q = b.gto_pref[l * n + k] * exp(-gam * r2);
-                        r0[k * L1 + l] = w * q;
-                        r1[k * L1 + l] = (w * (-2.0 * gam) + dwr) * q;
+                        B[l] = b.poly_B0[k * L1 + l];
+                        Bh[l] = b.poly_Bh0[k * L1 + l];
                     }
-            } else {
-                for (int k = 0; k < n; ++k) {
-                    double B[L1], Bh[L1];
-                    if (r < 1e-4) {
+                } else {
+                    double I[L1 + 1], J[L1 + 1];
 #pragma unroll
-                        for (int l = 0; l < L1; ++l) {
-                            B[l] = b.poly_B0[k * L1 + l];
-                            Bh[l] = b.poly_Bh0[k * L1 + l];
-                        }
-                    } else {
-                        double I[L1 + 1], J[L1 + 1];
-#pragma unroll
-                        for (int l = 0; l <= L1; ++l) I[l] = J[l] = 0.0;
-                        const int Q = b.quad_n;
-                        poly_levels(b, r, L1, [&](int q, int l, double P) {
-                            const double gq = b.poly_gss[k * Q + q];
-                            if (l < L1) I[l] += P * gq;
-                            if (l >= 1) J[l - 1] += P * gq * b.poly_rx[q];
-                        });
-                        double rl = 1.0;
-#pragma unroll
-                        for (int l = 0; l < L1; ++l) {
-                            B[l] = I[l] / rl;
-                            Bh[l] = (2.0 * b.eta) * (J[l] - r * I[l]) / (rl * r);
-                            rl *= r;
-                        }
-                    }
+                    for (int l = 0; l <= L1; ++l) I[l] = J[l] = 0.0;
+                    const int Q = b.quad_n;
+                    poly_levels(b, r, L1, [&](int q, int l, double P) {
+                        const double gq = b.poly_gss[k * Q + q];
+                        if (l < L1) I[l] += P * gq;
+                        if (l >= 1) J[l - 1] += P * gq * b.poly_rx[q];
+                    });
+                    double rl = 1.0;
 #pragma unroll
                     for (int l = 0; l < L1; ++l) {
-                        r0[k * L1 + l] = w * B[l];
-                        r1[k * L1 + l] = w * Bh[l] + dwr * B[l];
+                        B[l] = I[l] / rl;
+                        Bh[l] = (2.0 * b.eta) * (J[l] - r * I[l]) / (rl * r);
+                        rl *= r;
                     }
                 }
+#pragma unroll
+                for (int l = 0; l < L1; ++l) {
+                    r0[k * L1 + l] = w * B[l];
+                    r1[k * L1 + l] = w * Bh[l] + dwr * B[l];
+                }
             }
-            const double* Mrow = Ms + (size_t)s * n * L2;
+            const double* Mrow = Ms + (size_t)s * L2 * n;
             double gx = 0.0, gy = 0.0, gz = 0.0;
             solid_harmonics<LMAX, true, double>(rx, ry, rz, [&](int lm, double y, double yx, double yy, double yz) {
                 const int l = lm_l2(lm);
                 double h0 = 0.0, h1 = 0.0;
                 for (int k = 0; k < n; ++k) {
-                    const double mv = Mrow[k * L2 + lm];
+                    const double mv = Mrow[lm * n + k];
                     h0 += r0[k * L1 + l] * mv;
                     h1 += r1[k * L1 + l] * mv;
                 }
@@ -207,28 +330,65 @@ __global__ __launch_bounds__(64) void vjp_edge_kernel(cusoap_basis_t b, const do
     }
 }
 
+template <int LMAX, typename TM, bool MIXED>
+int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
+                   const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
+                   const int64_t* offsets, const int32_t* nbr, const void* M, double* grad, hipStream_t st) {
+    const size_t smem = sizeof(TM) * ((size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1) + 2 * 256 * (size_t)b.n_max) +
+                        (MIXED ? sizeof(double) * 256 * (size_t)b.n_max : 0);
+    if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: coefficient block too large for LDS");
+    auto kern = vjp_gto_kernel<LMAX, TM, MIXED>;
+    if (smem > 64 * 1024)
+        CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    prof_begin("vjp_edges", st);
+    kern<<<n_centers, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr,
+                                       (const TM*)M, grad);
+    prof_end("vjp_edges", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
 template <int LMAX>
-int launch_vjp_edges(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
+int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
                      const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
-                     const int64_t* offsets, const int32_t* nbr, const double* M, double* grad, hipStream_t st) {
+                     const int64_t* offsets, const int32_t* nbr, const void* M, double* grad, hipStream_t st) {
+    if (b.rbf == CUSOAP_RBF_GTO) {
+        if (prec == CUSOAP_PREC_F64)
+            return launch_vjp_gto<LMAX, double, false>(b, pos, n_atoms, atom_sp, shifts, centers, n_centers,
+                                                       center_attach, offsets, nbr, M, grad, st);
+        if (prec == CUSOAP_PREC_F32)
+            return launch_vjp_gto<LMAX, float, false>(b, pos, n_atoms, atom_sp, shifts, centers, n_centers,
+                                                      center_attach, offsets, nbr, M, grad, st);
+        return launch_vjp_gto<LMAX, float, true>(b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach,
+                                                 offsets, nbr, M, grad, st);
+    }
     const int L2 = (LMAX + 1) * (LMAX + 1);
     const size_t smem = sizeof(double) * ((size_t)b.n_species * b.n_max * L2 + 2 * 64 * (size_t)b.n_max * (LMAX + 1));
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: LDS tile too large");
-    if (b.rbf == CUSOAP_RBF_GTO) {
-        auto kern = vjp_edge_kernel<LMAX, CUSOAP_RBF_GTO>;
-        if (smem > 64 * 1024)
-            CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        prof_begin("vjp_edges", st);
-        kern<<<n_centers, 64, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr, M, grad);
-        prof_end("vjp_edges", st);
-    } else {
-        auto kern = vjp_edge_kernel<LMAX, CUSOAP_RBF_POLY>;
-        if (smem > 64 * 1024)
-            CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        prof_begin("vjp_edges", st);
-        kern<<<n_centers, 64, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr, M, grad);
-        prof_end("vjp_edges", st);
-    }
+    auto kern = vjp_poly_kernel<LMAX>;
+    if (smem > 64 * 1024)
+        CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    prof_begin("vjp_edges", st);
+    kern<<<n_centers, 64, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr,
+                                      (const double*)M, grad);
+    prof_end("vjp_edges", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
+template <typename TC, typename TG, typename TM>
+int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const int32_t* pair_feature, const void* gd,
+                   int gd_broadcast, int n_feat, int fold_beta, int n_centers, void* M, hipStream_t st) {
+    const int per = b.n_species * b.n_max * (b.l_max + 1) * (b.l_max + 1);
+    const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat);
+    if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: descriptor row too large for LDS");
+    auto kern = coeff_adjoint_kernel<TC, TG, TM>;
+    if (smem > 64 * 1024)
+        CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    prof_begin("vjp_adjoint", st);
+    kern<<<n_centers, 256, smem, st>>>(b, mode, (const TC*)coeffs, pair_feature, (const TG*)gd, gd_broadcast, n_feat,
+                                       fold_beta, (TM*)M);
+    prof_end("vjp_adjoint", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
 }
@@ -240,41 +400,36 @@ size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) {
     return align_up(sizeof(double) * (size_t)std::max(n_centers, 1) * b->n_species * b->n_max * L2, 256);
 }
 
-int vjp_impl(const cusoap_basis_t* b, int mode, const double* pos, int n_atoms, const int32_t* atom_sp,
+int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
              const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
-             const int64_t* offsets, const int32_t* nbr, const double* coeffs, const int32_t* pair_feature,
-             const void* gd, int gd_dtype, int gd_broadcast, int n_feat, void* ws, size_t ws_bytes, double* grad,
-             hipStream_t st) {
+             const int64_t* offsets, const int32_t* nbr, const void* coeffs, int coeffs_dtype,
+             const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast, int n_feat, void* ws,
+             size_t ws_bytes, double* grad, hipStream_t st) {
     (void)n_img;
     if (!b || b->n_max < 1 || b->n_max > kMaxN || b->l_max < 0 || b->l_max > kMaxL)
         return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: n_max must be 1..16 and l_max 0..9");
     if (mode == CUSOAP_MODE_PS && !pair_feature) return fail(CUSOAP_ERR_ARG, "vjp: pair_feature table missing");
+    if (prec < 0 || prec > 2) return fail(CUSOAP_ERR_ARG, "vjp: unknown precision");
     if (ws_bytes < vjp_ws_bytes(b, n_centers)) return fail(CUSOAP_ERR_WORKSPACE, "vjp: workspace too small");
     CUSOAP_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * (size_t)n_atoms, st));
     if (n_centers == 0) return CUSOAP_OK;
-    const int L2 = (b->l_max + 1) * (b->l_max + 1);
-    const int per = b->n_species * b->n_max * L2;
-    const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat);
-    if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: descriptor row too large for LDS");
-    double* M = (double*)ws;
-    if (gd_dtype == CUSOAP_DT_F64) {
-        auto kern = coeff_adjoint_kernel<double>;
-        if (smem > 64 * 1024)
-            CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        prof_begin("vjp_adjoint", st);
-        kern<<<n_centers, 256, smem, st>>>(*b, mode, coeffs, pair_feature, (const double*)gd, gd_broadcast, n_feat, M);
-        prof_end("vjp_adjoint", st);
+    if (b->rbf != CUSOAP_RBF_GTO) prec = CUSOAP_PREC_F64;   // the polynomial path is float64
+    const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED) ? 1 : 0;
+    int rc;
+#define CUSOAP_ADJ(TC, TG, TM) \
+    launch_adjoint<TC, TG, TM>(*b, mode, coeffs, pair_feature, gd, gd_broadcast, n_feat, fold, n_centers, ws, st)
+    const bool c64 = coeffs_dtype == CUSOAP_DT_F64, g64 = gd_dtype == CUSOAP_DT_F64;
+    if (prec == CUSOAP_PREC_F64) {
+        if (c64) rc = g64 ? CUSOAP_ADJ(double, double, double) : CUSOAP_ADJ(double, float, double);
+        else rc = g64 ? CUSOAP_ADJ(float, double, double) : CUSOAP_ADJ(float, float, double);
     } else {
-        auto kern = coeff_adjoint_kernel<float>;
-        if (smem > 64 * 1024)
-            CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        prof_begin("vjp_adjoint", st);
-        kern<<<n_centers, 256, smem, st>>>(*b, mode, coeffs, pair_feature, (const float*)gd, gd_broadcast, n_feat, M);
-        prof_end("vjp_adjoint", st);
+        if (c64) rc = g64 ? CUSOAP_ADJ(double, double, float) : CUSOAP_ADJ(double, float, float);
+        else rc = g64 ? CUSOAP_ADJ(float, double, float) : CUSOAP_ADJ(float, float, float);
     }
-    CUSOAP_LAUNCH_CHECK();
+#undef CUSOAP_ADJ
+    if (rc) return rc;
 #define CUSOAP_CASE(L) \
-    case L: return launch_vjp_edges<L>(*b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, offsets, nbr, M, grad, st);
+    case L: return launch_vjp_edges<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, offsets, nbr, ws, grad, st);
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)

@@ -2,41 +2,70 @@
 //
 // Restates _coefficients_gto / _coefficients_polynomial / _add_self_terms
 // (soap_generator.py:1294-1435, 821-850) and the Cacc accumulation of the
-// closed-form derivative path (soap_generator.py:2414-2423) as one pass:
+// closed-form derivative path (soap_generator.py:2414-2423) as one pass per
+// center:
 //
-//   one wavefront (64 lanes) per center; for each species s in turn the
-//   center's CSR edge segment is stream-compacted (ballot + popcount) into
-//   LDS tiles of 64 edges of that species.  Per tile, lane e evaluates its
-//   edge: real solid harmonics r^l Y_lm (angular type TY) and the radial
-//   factors (type TR) -- GTO primitives w * pref_kl * exp(-gamma_kl r^2), or
-//   the orthonormal polynomial-basis B_nl = w I_nl(r) / r^l -- into LDS.  The
-//   lanes then own (n, lm) accumulator slots in registers and sweep the tile.
-//   GTO accumulators hold primitive sums; the Loewdin beta_l contraction is
-//   applied once per center (S n^2 L^2 work instead of per edge).
+//   one 256-thread workgroup per center walks the center's CSR edge segment
+//   in tiles of 64 edges.  Phase A evaluates the tile: thread per edge for the
+//   real solid harmonics r^l Y_lm, thread per (edge, radial item) for the
+//   radial factors, both stored in LDS as [value][edge] rows (padded to 65 so
+//   that threads reading different rows hit different banks).  Phase B:
+//   wavefront w takes edges w, w+4, ... of the tile; each lane owns register
+//   accumulators for (2 radial n) x (4 consecutive lm of one l) x (up to 4
+//   species, chosen by a wavefront-uniform branch on the edge's species).
+//   The four wavefronts' partial sums are combined in LDS in a fixed order, so
+//   results are bitwise reproducible.
 //
-// The center's own atom (create path) is not an edge; its l=0 self term is
-// added at the end, as the reference does.
+// Arithmetic by precision:
+//   F64    harmonics, radial, accumulation in double; GTO accumulates the
+//          primitives w pref_kl exp(-gamma_kl r^2) and applies the Loewdin
+//          beta_l once per center (S n^2 L^2 work instead of per edge).
+//   F32    the same in float (the reference's algorithm in single precision:
+//          the beta contraction then loses digits at large n_max).
+//   MIXED  radial functions in double with beta applied per edge, so the
+//          orthonormal B_nl(r) (O(1), no cancellation left) is rounded to
+//          float once; harmonics and accumulation in float.
+// Polynomial basis: the quadrature radial integral is evaluated in double and
+// already orthonormal (no beta).
+//
+// create_path: edges with |r| <= 1e-12 are skipped and the l=0 self term is
+// added for index centers, as the reference does; otherwise every edge counts.
 #include "common.hpp"
 
 namespace cusoap {
 
 namespace {
 
-constexpr int kTile = 64;
+constexpr int kTE = 64;          // edges per tile
+constexpr int kTEP = kTE + 1;    // padded LDS row length
+constexpr int kThreads = 256;
 
-template <int LMAX>
-struct Dims {
-    static constexpr int L1 = LMAX + 1;
-    static constexpr int L2 = L1 * L1;
-    static constexpr int NSLOT = (kMaxN * L2 + 63) / 64;
-};
-
-__device__ __forceinline__ int lm_to_l(int lm) {
-    int l = (int)sqrtf((float)lm);
-    while (l * l > lm) --l;
-    while ((l + 1) * (l + 1) <= lm) ++l;
-    return l;
+__host__ __device__ constexpr int n_groups(int lmax) {
+    int g = 0;
+    for (int l = 0; l <= lmax; ++l) g += (2 * l + 1 + 3) / 4;
+    return g;
 }
+
+template <int PREC>
+struct CT;
+template <>
+struct CT<CUSOAP_PREC_F64> {
+    using TY = double;
+    using TA = double;
+    using TQ = double;
+};
+template <>
+struct CT<CUSOAP_PREC_F32> {
+    using TY = float;
+    using TA = float;
+    using TQ = float;
+};
+template <>
+struct CT<CUSOAP_PREC_MIXED> {
+    using TY = float;
+    using TA = float;
+    using TQ = double;
+};
 
 template <typename T>
 __device__ __forceinline__ T t_exp(T v);
@@ -45,180 +74,251 @@ __device__ __forceinline__ float t_exp<float>(float v) { return expf(v); }
 template <>
 __device__ __forceinline__ double t_exp<double>(double v) { return exp(v); }
 
-template <int LMAX, typename TY, typename TR, int RBF>
-__global__ __launch_bounds__(64) void coeff_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
-                                                   const int32_t* __restrict__ atom_sp,
-                                                   const double* __restrict__ shifts,
-                                                   const double* __restrict__ centers,
-                                                   const int32_t* __restrict__ self_sp,
-                                                   const int64_t* __restrict__ offsets,
-                                                   const int32_t* __restrict__ nbr, int create_path,
-                                                   TR* __restrict__ out) {
-    using D = Dims<LMAX>;
-    constexpr int L1 = D::L1, L2 = D::L2, NSLOT = D::NSLOT;
-    const int n = b.n_max, S = b.n_species;
-    const int NL = n * L1;
+template <int LMAX, int PREC, int RBF>
+__global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
+                                                         const int32_t* __restrict__ atom_sp,
+                                                         const double* __restrict__ shifts,
+                                                         const double* __restrict__ centers,
+                                                         const int32_t* __restrict__ self_sp,
+                                                         const int64_t* __restrict__ offsets,
+                                                         const int32_t* __restrict__ nbr, int create_path,
+                                                         typename CT<PREC>::TA* __restrict__ out) {
+    using TY = typename CT<PREC>::TY;
+    using TA = typename CT<PREC>::TA;
+    using TQ = typename CT<PREC>::TQ;
+    constexpr bool kBetaPerCenter = (RBF == CUSOAP_RBF_GTO) && (PREC != CUSOAP_PREC_MIXED);
+    constexpr int L1 = LMAX + 1, L2 = L1 * L1, NG = n_groups(LMAX);
+    const int n = b.n_max, S = b.n_species, NL = n * L1;
     extern __shared__ __align__(16) unsigned char smem[];
-    TR* Rsh = (TR*)smem;                              // [kTile][NL]
-    TR* accsh = Rsh + kTile * NL;                     // [n][L2]
-    TY* Ysh = (TY*)(accsh + n * L2);                  // [kTile][L2]
-    int32_t* esh = (int32_t*)(Ysh + kTile * L2);      // [2 * kTile] compacted ext indices
+    double* geo = (double*)smem;                   // [kTE][8]: rx ry rz r2 r w
+    int* spe = (int*)(geo + kTE * 8);              // [kTE] species, -1 = skip
+    TA* Rs = (TA*)(spe + kTE);                     // [NL][kTEP]
+    TY* Ys = (TY*)(Rs + NL * kTEP + 2);            // [L2][kTEP]
 
-    const int c = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int c = blockIdx.x, tid = threadIdx.x;
     const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
     const int64_t e0 = offsets[c], e1 = offsets[c + 1];
-    const int nslots = n * L2;
+    const int wv = tid >> 6, lane = tid & 63;
+    const int npair = (n + 1) / 2;
+    const int nslot_all = npair * NG;                 // (n pair, lm group) slots per species
+    const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
 
-    int slot_r[NSLOT], slot_y[NSLOT];
-#pragma unroll
-    for (int i = 0; i < NSLOT; ++i) {
-        const int t = lane + 64 * i;
-        const int nn = t / L2, lm = t - nn * L2;
-        slot_y[i] = lm;
-        slot_r[i] = nn * L1 + lm_to_l(lm);
-    }
-
-    for (int s = 0; s < S; ++s) {
-        TR acc[NSLOT];
-#pragma unroll
-        for (int i = 0; i < NSLOT; ++i) acc[i] = TR(0);
-        int64_t cursor = e0;
-        int cnt = 0;
-        while (cursor < e1 || cnt > 0) {
-            // ---- gather up to one tile of species-s edges
-            while (cnt < kTile && cursor < e1) {
-                const int64_t e = cursor + lane;
-                int ext = 0;
-                bool match = false;
-                if (e < e1) {
-                    ext = nbr[e];
-                    match = atom_sp[ext % n_atoms] == s;
+    for (int s0 = 0; s0 < S; s0 += 4) {
+        const int s1 = (s0 + 4 < S) ? s0 + 4 : S;
+        for (int q0 = 0; q0 < nslot_all; q0 += 64) {
+            // ---- this lane's slot: n pair p (n0 = 2p, n1 = 2p+1) x lm group g of degree l
+            const int q = q0 + lane;
+            const bool valid = q < nslot_all;
+            int n0 = 0, l = 0, lm0 = 0, cnt = 0;
+            bool has1 = false;
+            if (valid) {
+                const int pp = q / NG;
+                int g = q - pp * NG;
+                while (g >= (2 * l + 1 + 3) / 4) {
+                    g -= (2 * l + 1 + 3) / 4;
+                    ++l;
                 }
-                const uint64_t bal = __ballot(match);
-                const int pre = __popcll(bal & ((1ull << lane) - 1ull));
-                if (match) esh[cnt + pre] = ext;
-                cnt += __popcll(bal);
-                cursor += 64;
+                n0 = 2 * pp;
+                has1 = n0 + 1 < n;
+                lm0 = l * l + 4 * g;
+                const int left = 2 * l + 1 - 4 * g;
+                cnt = left < 4 ? left : 4;
+            }
+            const int r0i = (n0 * L1 + l) * kTEP, r1i = ((n0 + 1) * L1 + l) * kTEP, yi = lm0 * kTEP;
+            TA acc[4][2][4];
+#pragma unroll
+            for (int a4 = 0; a4 < 4; ++a4)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[a4][h][j] = TA(0);
+
+            for (int64_t t0 = e0; t0 < e1; t0 += kTE) {
+                const int nt = (int)((e1 - t0) < kTE ? (e1 - t0) : kTE);
+                // ---- A1: geometry
+                if (tid < nt) {
+                    const int ext = nbr[t0 + tid];
+                    const int img = ext / n_atoms, a = ext - img * n_atoms;
+                    const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
+                    const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
+                    const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
+                    const double r2 = (rx * rx + ry * ry) + rz * rz;
+                    const double r = sqrt(r2);
+                    double w = 1.0, dw = 0.0;
+                    if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
+                    double* g = geo + tid * 8;
+                    g[0] = rx; g[1] = ry; g[2] = rz; g[3] = r2; g[4] = r; g[5] = w;
+                    const int sp = atom_sp[a];
+                    spe[tid] = ((create_path && !(r > 1e-12)) || sp < s0 || sp >= s1) ? -1 : sp;
+                }
+                __syncthreads();
+                // ---- A2: harmonics (item per edge) and radial factors
+                int n_rad;
+                if (RBF == CUSOAP_RBF_POLY) n_rad = nt * n;
+                else if (kBetaPerCenter) n_rad = nt * NL;
+                else n_rad = nt * L1;
+                for (int item = tid; item < nt + n_rad; item += kThreads) {
+                    if (item < nt) {
+                        if (spe[item] < 0) continue;
+                        const double* g = geo + item * 8;
+                        TY* y = Ys + item;
+                        solid_harmonics<LMAX, false, TY>((TY)g[0], (TY)g[1], (TY)g[2],
+                                                         [&](int lm, TY v, TY, TY, TY) { y[lm * kTEP] = v; });
+                        continue;
+                    }
+                    const int it = item - nt;
+                    if (RBF == CUSOAP_RBF_GTO && kBetaPerCenter) {
+                        const int t = it / nt, e = it - t * nt;
+                        if (spe[e] < 0) continue;
+                        const int k = t / L1, ll = t - k * L1;
+                        const TQ pref = (TQ)b.gto_pref[ll * n + k], gam = (TQ)b.gto_gamma[ll * n + k];
+                        Rs[t * kTEP + e] = (TA)((TQ)geo[e * 8 + 5] * (pref * t_exp<TQ>(-gam * (TQ)geo[e * 8 + 3])));
+                    } else if (RBF == CUSOAP_RBF_GTO) {
+                        const int ll = it / nt, e = it - ll * nt;
+                        if (spe[e] < 0) continue;
+                        const double r2 = geo[e * 8 + 3], w = geo[e * 8 + 5];
+                        double qv[kMaxN];
+#pragma unroll
+                        for (int k = 0; k < kMaxN; ++k)
+                            qv[k] = k < n ? b.gto_pref[ll * n + k] * exp(-b.gto_gamma[ll * n + k] * r2) : 0.0;
+                        for (int nn = 0; nn < n; ++nn) {
+                            const double* beta = b.gto_beta + ((size_t)ll * n + nn) * n;
+                            double B = 0.0;
+#pragma unroll
+                            for (int k = 0; k < kMaxN; ++k)
+                                if (k < n) B += beta[k] * qv[k];
+                            Rs[(nn * L1 + ll) * kTEP + e] = (TA)(w * B);
+                        }
+                    } else {
+                        const int k = it / nt, e = it - k * nt;
+                        if (spe[e] < 0) continue;
+                        const double r = geo[e * 8 + 4], w = geo[e * 8 + 5];
+                        double I[L1];
+#pragma unroll
+                        for (int ll = 0; ll < L1; ++ll) I[ll] = 0.0;
+                        if (!create_path && r < 1e-4) {
+#pragma unroll
+                            for (int ll = 0; ll < L1; ++ll) I[ll] = b.poly_B0[k * L1 + ll];
+                        } else {
+                            const int Q = b.quad_n;
+                            poly_levels(b, r, LMAX, [&](int qq, int ll, double P) { I[ll] += P * b.poly_gss[k * Q + qq]; });
+                            double rl = 1.0;
+#pragma unroll
+                            for (int ll = 0; ll < L1; ++ll) {
+                                I[ll] /= rl;
+                                rl *= r;
+                            }
+                        }
+#pragma unroll
+                        for (int ll = 0; ll < L1; ++ll) Rs[(k * L1 + ll) * kTEP + e] = (TA)(w * I[ll]);
+                    }
+                }
+                __syncthreads();
+                // ---- B: wavefront wv sweeps edges wv, wv+4, ...; species is wavefront-uniform
+                if (valid) {
+                    for (int e = wv; e < nt; e += 4) {
+                        const int se = __builtin_amdgcn_readfirstlane(spe[e]);
+                        if (se < 0) continue;
+                        const TA rv0 = Rs[r0i + e];
+                        const TA rv1 = has1 ? Rs[r1i + e] : TA(0);
+                        TA yv[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) yv[j] = j < cnt ? (TA)Ys[yi + j * kTEP + e] : TA(0);
+                        const int sl = se - s0;
+#pragma unroll
+                        for (int a4 = 0; a4 < 4; ++a4) {
+                            if (sl == a4) {
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    acc[a4][0][j] += rv0 * yv[j];
+                                    acc[a4][1][j] += rv1 * yv[j];
+                                }
+                            }
+                        }
+                    }
+                }
                 __syncthreads();
             }
-            const int ntile = cnt < kTile ? cnt : kTile;
-            // ---- evaluate the tile's edges
-            if (lane < ntile) {
-                const int ext = esh[lane];
-                const int img = ext / n_atoms, a = ext - img * n_atoms;
-                const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
-                const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
-                const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
-                const double r2 = (rx * rx + ry * ry) + rz * rz;
-                const double r = sqrt(r2);
-                double w = 1.0, dw = 0.0;
-                if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
-                TY* Yrow = Ysh + lane * L2;
-                solid_harmonics<LMAX, false, TY>((TY)rx, (TY)ry, (TY)rz,
-                                                 [&](int lm, TY v, TY, TY, TY) { Yrow[lm] = v; });
-                TR* Rrow = Rsh + lane * NL;
-                if (RBF == CUSOAP_RBF_GTO) {
-                    const TR tr2 = (TR)r2, tw = (TR)w;
-                    for (int k = 0; k < n; ++k) {
+            // ---- combine the four wavefronts' partial sums in a fixed order (LDS, aliases the tile)
+            TA* part = Rs;                                  // [4 species][n][L2]
+            for (int w2 = 0; w2 < 4; ++w2) {
+                if (wv == w2 && valid) {
 #pragma unroll
-                        for (int l = 0; l < L1; ++l) {
-                            const TR pref = (TR)b.gto_pref[l * n + k];
-                            const TR gam = (TR)b.gto_gamma[l * n + k];
-                            Rrow[k * L1 + l] = tw * (pref * t_exp<TR>(-gam * tr2));
-                        }
-                    }
-                } else {
-                    for (int i = 0; i < NL; ++i) Rrow[i] = TR(0);
-                    const bool use_limit = !create_path && r < 1e-4;
-                    if (use_limit) {
-                        for (int k = 0; k < n; ++k)
+                    for (int a4 = 0; a4 < 4; ++a4) {
+                        if (s0 + a4 >= s1) continue;
 #pragma unroll
-                            for (int l = 0; l < L1; ++l) Rrow[k * L1 + l] = (TR)(w * b.poly_B0[k * L1 + l]);
-                    } else {
-                        const int Q = b.quad_n;
-                        poly_levels(b, r, LMAX, [&](int q, int l, double P) {
-                            for (int k = 0; k < n; ++k) Rrow[k * L1 + l] += (TR)(P * b.poly_gss[k * Q + q]);
-                        });
-                        double rl = 1.0;
+                        for (int h = 0; h < 2; ++h) {
+                            if (h == 1 && !has1) continue;
 #pragma unroll
-                        for (int l = 0; l < L1; ++l) {
-                            for (int k = 0; k < n; ++k) Rrow[k * L1 + l] = (TR)(w * ((double)Rrow[k * L1 + l] / rl));
-                            rl *= r;
+                            for (int j = 0; j < 4; ++j) {
+                                if (j >= cnt) continue;
+                                TA* dst = part + (a4 * n + n0 + h) * L2 + lm0 + j;
+                                *dst = (w2 == 0) ? acc[a4][h][j] : *dst + acc[a4][h][j];
+                            }
                         }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-            // ---- accumulate
-            for (int e = 0; e < ntile; ++e) {
-                const TR* Rr = Rsh + e * NL;
-                const TY* Yr = Ysh + e * L2;
-#pragma unroll
-                for (int i = 0; i < NSLOT; ++i) {
-                    if (lane + 64 * i < nslots) acc[i] += Rr[slot_r[i]] * (TR)Yr[slot_y[i]];
-                }
+            // ---- write this pass's outputs (raw primitive sums when beta is applied per center)
+            if (wv == 0 && valid) {
+                for (int a4 = 0; a4 < s1 - s0; ++a4)
+                    for (int h = 0; h < 2; ++h) {
+                        if (h == 1 && !has1) continue;
+                        const int nn = n0 + h;
+                        TA* dst = out + (((size_t)c * S + s0 + a4) * n + nn) * L2 + lm0;
+                        for (int j = 0; j < cnt; ++j) {
+                            TA v = part[(a4 * n + nn) * L2 + lm0 + j];
+                            if (!kBetaPerCenter && s0 + a4 == self_s && lm0 + j == 0) v += (TA)b.self_l0[nn];
+                            dst[j] = v;
+                        }
+                    }
             }
-            __syncthreads();
-            // ---- keep the remainder of the gathered list
-            const int rem = cnt - ntile;
-            int keep = 0;
-            if (lane < rem) keep = esh[kTile + lane];
-            __syncthreads();
-            if (lane < rem) esh[lane] = keep;
-            cnt = rem;
             __syncthreads();
         }
-        // ---- finalise species s
-        const size_t obase = ((size_t)c * S + s) * (size_t)nslots;
-        const bool self_here = create_path && self_sp[c] == s;
-        if (RBF == CUSOAP_RBF_GTO) {
-#pragma unroll
-            for (int i = 0; i < NSLOT; ++i)
-                if (lane + 64 * i < nslots) accsh[lane + 64 * i] = acc[i];
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < NSLOT; ++i) {
-                const int t = lane + 64 * i;
-                if (t < nslots) {
-                    const int nn = t / L2, lm = slot_y[i], l = slot_r[i] - nn * L1;
-                    const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
-                    TR v = TR(0);
-                    for (int k = 0; k < n; ++k) v += (TR)beta[k] * accsh[k * L2 + lm];
-                    if (self_here && lm == 0) v += (TR)b.self_l0[nn];
-                    out[obase + t] = v;
-                }
-            }
-            __syncthreads();
-        } else {
-#pragma unroll
-            for (int i = 0; i < NSLOT; ++i) {
-                const int t = lane + 64 * i;
-                if (t < nslots) {
-                    TR v = acc[i];
-                    if (self_here && slot_y[i] == 0) v += (TR)b.self_l0[t / L2];
-                    out[obase + t] = v;
-                }
-            }
+    }
+    if (kBetaPerCenter) {
+        // ---- Loewdin contraction of the primitive sums, once per center
+        __threadfence_block();
+        TA* Cs = Rs;                                        // [S][n][L2]
+        const int per = S * n * L2;
+        TA* oc = out + (size_t)c * per;
+        for (int i = tid; i < per; i += kThreads) Cs[i] = oc[i];
+        __syncthreads();
+        for (int o = tid; o < per; o += kThreads) {
+            const int s = o / (n * L2), rem = o - s * n * L2;
+            const int nn = rem / L2, lm = rem - nn * L2;
+            int ll = 0;
+            while ((ll + 1) * (ll + 1) <= lm) ++ll;
+            const double* beta = b.gto_beta + ((size_t)ll * n + nn) * n;
+            TA v = TA(0);
+            for (int k = 0; k < n; ++k) v += (TA)beta[k] * Cs[(s * n + k) * L2 + lm];
+            if (s == self_s && lm == 0) v += (TA)b.self_l0[nn];
+            oc[o] = v;
         }
     }
 }
 
-template <int LMAX, typename TY, typename TR, int RBF>
+template <int LMAX, int PREC, int RBF>
 int launch_coeff(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
                  const double* shifts, const double* centers, int n_centers, const int32_t* self_sp,
                  const int64_t* offsets, const int32_t* nbr, int create_path, void* out, hipStream_t st) {
-    using D = Dims<LMAX>;
-    const int n = b.n_max;
-    const size_t smem = sizeof(TR) * ((size_t)kTile * n * D::L1 + (size_t)n * D::L2) +
-                        sizeof(TY) * (size_t)kTile * D::L2 + sizeof(int32_t) * 2 * kTile;
+    using TY = typename CT<PREC>::TY;
+    using TA = typename CT<PREC>::TA;
+    constexpr int L1 = LMAX + 1, L2 = L1 * L1;
+    const int n = b.n_max, NL = n * L1;
+    const size_t tile = sizeof(double) * kTE * 8 + sizeof(int) * kTE + sizeof(TA) * ((size_t)NL * kTEP + 2) +
+                        sizeof(TY) * (size_t)L2 * kTEP;
+    const size_t cs = sizeof(TA) * (size_t)std::max(b.n_species, 4) * n * L2 + sizeof(double) * kTE * 8 +
+                      sizeof(int) * kTE;
+    const size_t smem = tile > cs ? tile : cs;
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: LDS tile too large");
-    auto kern = coeff_kernel<LMAX, TY, TR, RBF>;
+    auto kern = coeff_kernel<LMAX, PREC, RBF>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("coefficients", st);
-    kern<<<n_centers, 64, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, self_sp, offsets, nbr,
-                                      create_path, (TR*)out);
+    kern<<<n_centers, kThreads, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, self_sp, offsets, nbr,
+                                            create_path, (TA*)out);
     prof_end("coefficients", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -230,13 +330,13 @@ int dispatch_prec(const cusoap_basis_t& b, int prec, const double* pos, int n_at
                   const int64_t* offsets, const int32_t* nbr, int create_path, void* out, hipStream_t st) {
 #define CUSOAP_COEFF_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st
     if (b.rbf == CUSOAP_RBF_GTO) {
-        if (prec == CUSOAP_PREC_F64) return launch_coeff<LMAX, double, double, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
-        if (prec == CUSOAP_PREC_F32) return launch_coeff<LMAX, float, float, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
-        return launch_coeff<LMAX, float, double, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
+        if (prec == CUSOAP_PREC_F64) return launch_coeff<LMAX, CUSOAP_PREC_F64, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
+        if (prec == CUSOAP_PREC_F32) return launch_coeff<LMAX, CUSOAP_PREC_F32, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
+        return launch_coeff<LMAX, CUSOAP_PREC_MIXED, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
     }
-    if (prec == CUSOAP_PREC_F64) return launch_coeff<LMAX, double, double, CUSOAP_RBF_POLY>(CUSOAP_COEFF_ARGS);
-    if (prec == CUSOAP_PREC_F32) return launch_coeff<LMAX, float, float, CUSOAP_RBF_POLY>(CUSOAP_COEFF_ARGS);
-    return launch_coeff<LMAX, float, double, CUSOAP_RBF_POLY>(CUSOAP_COEFF_ARGS);
+    if (prec == CUSOAP_PREC_F64) return launch_coeff<LMAX, CUSOAP_PREC_F64, CUSOAP_RBF_POLY>(CUSOAP_COEFF_ARGS);
+    if (prec == CUSOAP_PREC_F32) return launch_coeff<LMAX, CUSOAP_PREC_F32, CUSOAP_RBF_POLY>(CUSOAP_COEFF_ARGS);
+    return launch_coeff<LMAX, CUSOAP_PREC_MIXED, CUSOAP_RBF_POLY>(CUSOAP_COEFF_ARGS);
 #undef CUSOAP_COEFF_ARGS
 }
 

@@ -118,6 +118,91 @@ __device__ __forceinline__ void solid_harmonics(T x, T y, T z, Put&& put) {
     }
 }
 
+
+// l-major variant of the same harmonics: on_l(l) once at the start of degree l,
+// then put(l, lm, v, gx, gy, gz) for m = -l..l (lm ascending).  Keeps the degree
+// l-1 and l-2 columns of the recurrence for every m in registers.
+template <int LMAX, bool GRAD, typename T, typename OnL, typename Put>
+__device__ __forceinline__ void solid_harmonics_lmajor(T x, T y, T z, OnL&& on_l, Put&& put) {
+    const T r2 = (x * x + y * y) + z * z;
+    T cm[LMAX + 1], sm[LMAX + 1];
+    cm[0] = T(1);
+    sm[0] = T(0);
+#pragma unroll
+    for (int m = 1; m <= LMAX; ++m) {
+        cm[m] = x * cm[m - 1] - y * sm[m - 1];
+        sm[m] = x * sm[m - 1] + y * cm[m - 1];
+    }
+    T q1[LMAX + 1], q1z[LMAX + 1], q1r[LMAX + 1], q2[LMAX + 1], q2z[LMAX + 1], q2r[LMAX + 1];
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) q1[m] = q1z[m] = q1r[m] = q2[m] = q2z[m] = q2r[m] = T(0);
+#pragma unroll
+    for (int l = 0; l <= LMAX; ++l) {
+        T q[LMAX + 1], qz[LMAX + 1], qr[LMAX + 1];
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            if (m == l) {
+                q[m] = T(kDoubleFact[m]);
+                qz[m] = T(0);
+                qr[m] = T(0);
+            } else if (m == l - 1) {
+                q[m] = T(2 * m + 1) * z * q1[m];
+                qz[m] = T(2 * m + 1) * q1[m];
+                qr[m] = T(0);
+            } else {
+                const T a = T(2 * l - 1), bb = T(l + m - 1), inv = T(1.0 / (l - m));
+                q[m] = (a * z * q1[m] - bb * r2 * q2[m]) * inv;
+                qz[m] = (a * (q1[m] + z * q1z[m]) - bb * r2 * q2z[m]) * inv;
+                qr[m] = (a * z * q1r[m] - bb * (q2[m] + r2 * q2r[m])) * inv;
+            }
+        }
+        const int c0 = l * l + l;
+        on_l(l);
+#pragma unroll
+        for (int mm = -l; mm <= l; ++mm) {
+            const int am = mm < 0 ? -mm : mm;
+            const T K = T(kSHNorm[l][am]);
+            T v, gx = T(0), gy = T(0), gz = T(0);
+            if (am == 0) {
+                v = K * q[0];
+                if (GRAD) {
+                    gx = K * T(2) * x * qr[0];
+                    gy = K * T(2) * y * qr[0];
+                    gz = K * (qz[0] + T(2) * z * qr[0]);
+                }
+            } else {
+                const T dqx = T(2) * x * qr[am], dqy = T(2) * y * qr[am], dqz = qz[am] + T(2) * z * qr[am];
+                const T fm = T(am);
+                if (mm > 0) {
+                    v = K * q[am] * cm[am];
+                    if (GRAD) {
+                        gx = K * (dqx * cm[am] + q[am] * fm * cm[am - 1]);
+                        gy = K * (dqy * cm[am] - q[am] * fm * sm[am - 1]);
+                        gz = K * dqz * cm[am];
+                    }
+                } else {
+                    v = K * q[am] * sm[am];
+                    if (GRAD) {
+                        gx = K * (dqx * sm[am] + q[am] * fm * sm[am - 1]);
+                        gy = K * (dqy * sm[am] + q[am] * fm * cm[am - 1]);
+                        gz = K * dqz * sm[am];
+                    }
+                }
+            }
+            put(l, c0 + mm, v, gx, gy, gz);
+        }
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            q2[m] = q1[m];
+            q2z[m] = q1z[m];
+            q2r[m] = q1r[m];
+            q1[m] = q[m];
+            q1z[m] = qz[m];
+            q1r[m] = qr[m];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // weighting (w(r) and dw/dr; w0 and zero slope below 1e-12)
 // ---------------------------------------------------------------------------

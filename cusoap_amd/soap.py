@@ -530,7 +530,7 @@ class SOAP:
         return _NL(offsets, nbr, int(total.value), int(mx.value))
 
     def _coeff_dtype(self, precision: str) -> torch.dtype:
-        return torch.float32 if precision == "float32" else torch.float64
+        return torch.float64 if precision == "float64" else torch.float32
 
     def _coefficients(self, x: _Ctx, nl: _NL, create_path: bool, precision: Optional[str] = None) -> torch.Tensor:
         precision = precision or self.precision
@@ -664,12 +664,11 @@ class SOAP:
             inv[np.asarray(idx)] = np.arange(n_inc, dtype=np.int32)
         closed = (not self.periodic) and self._weighting is None and not attach
         nl_keep = self._neighbors(x, drop_self=False)
-        prec = "float64" if self.precision == "float32" else self.precision
-        if closed:
-            coeffs = self._coefficients(x, nl_keep, create_path=False, precision=prec)
-        else:
-            nl_drop = self._neighbors(x, drop_self=True)
-            coeffs = self._coefficients(x, nl_drop, create_path=True, precision=prec)
+        # the dense Jacobian kernels take float64 coefficients
+        prec = "mixed" if self.precision == "mixed" else "float64"
+        coeffs = self._coefficients(x, nl_keep, create_path=not closed, precision=prec)
+        if coeffs.dtype != torch.float64:
+            coeffs = coeffs.double()
         mode = _lib.MODE_CC if self.average == "cc" else _lib.MODE_PS
         n_feat = self.get_number_of_features()
         deriv = torch.empty((x.n_centers, n_inc, 3, n_feat), dtype=self.dtype, device=self.device)
@@ -772,6 +771,16 @@ class SOAP:
         x = self._context(system, centers)
         return self._vjp_ctx(x, grad_desc, centers, attach)
 
+    def _vjp_coefficients(self, x: _Ctx, nl_keep: _NL) -> torch.Tensor:
+        """Coefficients for the descriptor a derivative differentiates, on the kept-self-edge list.
+
+        Closed-form convention (non-periodic, unweighted): the self edge counts as
+        an edge.  Otherwise the create() convention (self edge skipped, self term
+        added), as the reference's autograd path differentiates create().
+        """
+        closed = (not self.periodic) and self._weighting is None
+        return self._coefficients(x, nl_keep, create_path=not closed)
+
     def _vjp_ctx(self, x: _Ctx, grad_desc, centers, attach, coeffs=None, nl_keep=None):
         lib = _lib.load()
         b = self._basis()
@@ -792,14 +801,7 @@ class SOAP:
         if nl_keep is None:
             nl_keep = self._neighbors(x, drop_self=False)
         if coeffs is None:
-            closed = (not self.periodic) and self._weighting is None
-            prec = "float64" if self.precision == "float32" else self.precision
-            if closed:
-                coeffs = self._coefficients(x, nl_keep, create_path=False, precision=prec)
-            else:
-                coeffs = self._coefficients(x, self._neighbors(x, drop_self=True), create_path=True, precision=prec)
-        if coeffs.dtype != torch.float64:
-            coeffs = coeffs.double()
+            coeffs = self._vjp_coefficients(x, nl_keep)
         nbytes = C.c_size_t(0)
         _lib.check(lib.cusoap_vjp_workspace_size(C.byref(b), x.n_centers, C.byref(nbytes)), "vjp_workspace_size")
         ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
@@ -807,22 +809,19 @@ class SOAP:
         att = self._attach_array(x, centers, attach)
         mode = _lib.MODE_CC if self.average == "cc" else _lib.MODE_PS
         _lib.check(lib.cusoap_vjp(
-            C.byref(b), mode, x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(), x.shifts.data_ptr(), x.n_img,
-            x.centers.data_ptr(), x.n_centers, att.data_ptr() if att is not None else None,
-            nl_keep.offsets.data_ptr(), nl_keep.nbr.data_ptr(), coeffs.data_ptr(), self._keep["pf"].data_ptr(),
-            g.data_ptr(), _lib.DT_F64 if g.dtype == torch.float64 else _lib.DT_F32, bcast, n_feat,
-            ws.data_ptr(), ws.numel(), grad.data_ptr(), self._stream()), "vjp")
+            C.byref(b), mode, _PRECISIONS[self.precision], x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(),
+            x.shifts.data_ptr(), x.n_img, x.centers.data_ptr(), x.n_centers,
+            att.data_ptr() if att is not None else None, nl_keep.offsets.data_ptr(), nl_keep.nbr.data_ptr(),
+            coeffs.data_ptr(), _lib.DT_F64 if coeffs.dtype == torch.float64 else _lib.DT_F32,
+            self._keep["pf"].data_ptr(), g.data_ptr(), _lib.DT_F64 if g.dtype == torch.float64 else _lib.DT_F32,
+            bcast, n_feat, ws.data_ptr(), ws.numel(), grad.data_ptr(), self._stream()), "vjp")
         return grad
 
     def descriptor_and_vjp(self, system, grad_desc, centers=None, attach=False):
-        """One neighbour search and coefficient pass for both the descriptor and its adjoint."""
+        """Descriptor and its adjoint from one neighbour search and one coefficient pass."""
         x = self._context(system, centers)
         nl_keep = self._neighbors(x, drop_self=False)
-        closed = (not self.periodic) and self._weighting is None
-        if closed:
-            coeffs = self._coefficients(x, nl_keep, create_path=False)
-        else:
-            coeffs = self._coefficients(x, self._neighbors(x, drop_self=True), create_path=True)
+        coeffs = self._vjp_coefficients(x, nl_keep)
         desc = self._descriptor(coeffs)
         return desc, self._vjp_ctx(x, grad_desc, centers, attach, coeffs=coeffs, nl_keep=nl_keep)
 

@@ -41,10 +41,11 @@ enum {
 
 enum { CUSOAP_RBF_GTO = 0, CUSOAP_RBF_POLY = 1 };
 enum { CUSOAP_W_NONE = 0, CUSOAP_W_POLY = 1, CUSOAP_W_POW = 2, CUSOAP_W_EXP = 3 };
-/* Arithmetic of the coefficient stage.  F64: everything in double.  MIXED:
- * harmonics in float, radial functions / accumulation / beta contraction in
- * double.  F32: everything in float (the Loewdin contraction then loses
- * digits at large n_max). */
+/* Arithmetic of the coefficient and adjoint stages.  F64: everything in
+ * double.  MIXED: radial functions (exponentials and the Loewdin beta
+ * contraction) in double per edge, harmonics and accumulation in float.
+ * F32: everything in float (the Loewdin contraction then loses digits at
+ * large n_max). */
 enum { CUSOAP_PREC_F64 = 0, CUSOAP_PREC_F32 = 1, CUSOAP_PREC_MIXED = 2 };
 enum { CUSOAP_DT_F64 = 0, CUSOAP_DT_F32 = 1 };
 enum { CUSOAP_MODE_PS = 0, CUSOAP_MODE_CC = 1 };
@@ -129,8 +130,8 @@ int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_
  * create_path != 0: edges with |r| <= 1e-12 are skipped and self_l0 is added
  * for centers with center_self_species >= 0 (index centers); otherwise every
  * edge is kept (the closed-form derivative path).
- * coeffs: (n_centers, n_species, n_max, (l_max+1)^2), float64 for PREC_F64 /
- * PREC_MIXED, float32 for PREC_F32. */
+ * coeffs: (n_centers, n_species, n_max, (l_max+1)^2), float64 for PREC_F64,
+ * float32 for PREC_MIXED / PREC_F32. */
 int cusoap_coefficients(const cusoap_basis_t* basis, int32_t precision,
                         const double* positions, int32_t n_atoms, const int32_t* atom_species,
                         const double* shifts, int32_t n_img,
@@ -177,16 +178,20 @@ int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
  *      reference contracts its dense Jacobian, soap_generator.py:2270-2626) ---
  * grad_pos (n_atoms, 3) float64 receives sum_{c,f} grad_desc[c,f] d desc[c,f] / d x
  * (it is zeroed first).  grad_desc: (n_centers, n_feat) in gd_dtype, or with
- * gd_broadcast != 0 a single (n_feat) row used for every center.  coeffs as for
- * the Jacobian.  center_attach as above. */
+ * gd_broadcast != 0 a single (n_feat) row used for every center.  coeffs:
+ * (n_centers, S, n, L2) in coeffs_dtype, from cusoap_coefficients on the same
+ * centers.  precision: CUSOAP_PREC_* of the edge pass (the polynomial basis
+ * always runs in float64).  pair_feature: (l_max+1, S n, S n) int32 feature
+ * index of each channel pair or -1 (power spectrum only).  center_attach as
+ * for the Jacobian.  Forces are accumulated with float64 atomics. */
 int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes);
-int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode,
+int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
                const double* positions, int32_t n_atoms, const int32_t* atom_species,
                const double* shifts, int32_t n_img,
                const double* centers, int32_t n_centers, const int32_t* center_attach,
                const int64_t* offsets, const int32_t* nbr,
-               const double* coeffs, const int32_t* pair_feature, const void* grad_desc,
-               int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
+               const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
+               const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
                void* ws, size_t ws_bytes, double* grad_pos, void* stream);
 
 /* ---- Loewdin orthonormalisation S^-1/2 in float64 (replaces

@@ -275,3 +275,30 @@ def test_default_basis_create_matches_oracle_cfg1():
     o = so.OracleSOAP(**W300)
     g = SOAP(device="cuda:0", dtype="float64", **W300)   # beta from the device Jacobi kernel
     assert colerr(g.create(s), o.create(s)) < RTOL64
+
+
+@pytest.mark.parametrize("prec,tol", [("mixed", 1e-5), ("float32", 5e-3)])
+def test_vjp_precision_forms(prec, tol):
+    s = systems.water_cluster_300(seed=11)
+    p = dict(r_cut=6.0, n_max=6, l_max=4, sigma=0.5, species=[1, 8])
+    o = so.OracleSOAP(**p)
+    g = SOAP(device="cuda:0", dtype="float64", precision=prec, _betas=o.betas, **p)
+    w = np.random.default_rng(2).normal(size=o.get_number_of_features())
+    say("vjp precision", prec)
+    d, F = g.descriptor_and_vjp(s, torch.tensor(w))
+    Fr = o.vjp(s, w[None, :].repeat(300, 0))
+    err = np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
+    say("  force rel err", err, "desc err", colerr(d, o.create(s)))
+    assert err < tol
+    assert colerr(d, o.create(s)) < tol
+
+
+def test_descriptor_and_vjp_periodic_weighted_f64():
+    s = systems.water((3, 3, 3), spacing=3.1, seed=12, cell_length=9.3)
+    p = dict(r_cut=4.0, n_max=4, l_max=3, sigma=0.5, species=[1, 8], periodic=True,
+             weighting=dict(function="pow", r0=1.0, c=1.0, d=1.0, m=2.0, w0=0.5))
+    o, g = pair(p)
+    G = np.random.default_rng(3).normal(size=(81, o.get_number_of_features()))
+    d, F = g.descriptor_and_vjp(s, torch.tensor(G))
+    np.testing.assert_allclose(F.cpu().numpy(), o.vjp(s, G), atol=1e-10 * np.abs(o.vjp(s, G)).max())
+    assert colerr(d, o.create(s)) < RTOL64
