@@ -42,7 +42,11 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="float64", choices=["float64", "mixed", "float32"])
     ap.add_argument("--config", default="h2o_1000", choices=["h2o_1000", "water_box", "hcno_box", "h2o_1m"])
-    ap.add_argument("--compare", default="", help="comma list of precisions to compare against float64 (large configs)")
+    ap.add_argument("--radial", default="exact", choices=["exact", "learned"],
+                    help="radial functions of the large configs: exact, or the learned model (trained first)")
+    ap.add_argument("--compare", default="",
+                    help="comma list of forms to compare against float64 (large configs): a precision, "
+                         "optionally with ':learned'")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-centers", type=int, default=48)
     ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
@@ -202,13 +206,24 @@ def bench_adjoint(args, dev):
     n_env = system["positions"].shape[0]
     sys_dev = dict(system, positions=torch.tensor(system["positions"], device=dev))
     soaps = {}
+    train = {}
 
-    def soap_for(prec):
-        if prec not in soaps:
-            soaps[prec] = SOAP(device=dev, dtype="float32", precision=prec, **params)
-        return soaps[prec]
+    def soap_for(prec, radial="exact"):
+        key = (prec, radial)
+        if key not in soaps:
+            if radial == "learned":
+                t1 = time.perf_counter()
+                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, radial="learned", **params)
+                sur = soaps[key].surrogate
+                train.update({"radial_training_s": round(time.perf_counter() - t1, 2),
+                              "radial_gate": [int(u) for u in sur.use_pair],
+                              "radial_heldout_value_err": float(sur.errors["value"].max()),
+                              "radial_heldout_slope_err": float(sur.errors["slope"].max())})
+            else:
+                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, **params)
+        return soaps[key]
 
-    soap = soap_for(args.precision)
+    soap = soap_for(args.precision, args.radial)
     n_feat = soap.get_number_of_features()
     w = torch.tensor(np.random.default_rng(12345).normal(size=n_feat), device=dev)
 
@@ -237,23 +252,26 @@ def bench_adjoint(args, dev):
     out = {"metric": METRIC, "value": round(n_env * args.steps / dt, 1), "unit": "atom-envs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": {"float64": "f64", "mixed": "f64+f32", "float32": "f32"}[args.precision],
+           "dtype": ("f32" if args.radial == "learned" and args.precision != "float64" else
+                     {"float64": "f64", "mixed": "f64+f32", "float32": "f32"}[args.precision]),
            "data": "synthetic",
            "config": {"workload": desc_txt, "n_atoms": n_env, "n_features": n_feat, "precision": args.precision,
+                      "radial": args.radial, **train,
                       "system_generation_s": round(t_gen, 1), "parallelism": "single GPU"},
            "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
     if args.compare:
         ref_d = d.double()
         ref_F = F
-        if args.precision != "float64":
+        if args.precision != "float64" or args.radial != "exact":
             ref_d, ref_F = step(soap_for("float64"))
             ref_d = ref_d.double()
         errs = {}
-        for prec in [p for p in args.compare.split(",") if p]:
-            dd, FF = step(soap_for(prec))
+        for form in [p for p in args.compare.split(",") if p]:
+            prec, _, rad = form.partition(":")
+            dd, FF = step(soap_for(prec, rad or "exact"))
             dd = dd.double()
             sc = ref_d.abs().amax(0).clamp_min(1e-300)
-            errs[prec] = {"desc_max_col_rel": float(((dd - ref_d).abs().amax(0) / sc).max()),
+            errs[form] = {"desc_max_col_rel": float(((dd - ref_d).abs().amax(0) / sc).max()),
                           "force_max_rel": float((FF - ref_F).abs().max() / ref_F.abs().max())}
         out["errors_vs_gpu_float64"] = errs
     print(json.dumps(out), flush=True)

@@ -29,7 +29,7 @@ class Basis(C.Structure):
                 ("w_r0", f64), ("w_c", f64), ("w_d", f64), ("w_m", f64), ("w0", f64),
                 ("gto_gamma", p), ("gto_pref", p), ("gto_beta", p),
                 ("poly_rx", p), ("poly_rxw", p), ("poly_gss", p), ("poly_B0", p), ("poly_Bh0", p),
-                ("self_l0", p)]
+                ("self_l0", p), ("rad_table", p), ("rad_intervals", i32), ("rad_rmax", f64)]
 
 
 class NLPlan(C.Structure):
@@ -53,9 +53,18 @@ _SIGS = {
     "cusoap_jacobian_dense": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i64, p, p, i32,
                               p, i32, p, sz, p, i32, i32, p],
     "cusoap_vjp_workspace_size": [C.POINTER(Basis), i32, C.POINTER(sz)],
-    "cusoap_vjp": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, i32, p, p, i32, i32, i32,
-                   p, sz, p, p],
+    "cusoap_vjp": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32, i32,
+                   i32, p, sz, p, p],
     "cusoap_lowdin_invsqrt_f64": [p, i32, i32, f64, p, p],
+    "cusoap_radial_exact": [C.POINTER(Basis), p, i32, p, p, p],
+    "cusoap_mlp_eval": [p, i32, i32, i32, p, i32, p, p, p],
+    "cusoap_mlp_train_workspace_size": [i32, i32, i32, i32, C.POINTER(sz)],
+    "cusoap_mlp_loss_grad": [p, i32, i32, i32, p, p, p, i32, i64, p, p, i32, C.c_uint32, i64, p, sz, p, p, p],
+    "cusoap_adam_step": [p, p, p, p, i64, C.c_float, C.c_float, C.c_float, C.c_float, i64, p],
+    "cusoap_mlp_refit_workspace_size": [i32, i32, i32, i32, C.POINTER(sz)],
+    "cusoap_mlp_refit_output": [p, i32, i32, i32, p, p, p, i32, i64, p, f64, p, sz, p],
+    "cusoap_radial_table_workspace_size": [C.POINTER(Basis), i32, C.POINTER(sz)],
+    "cusoap_radial_table": [C.POINTER(Basis), p, i32, p, p, f64, i32, f64, p, sz, p, p],
     "cusoap_profile_get": [C.c_char_p, C.POINTER(f64), C.POINTER(i32)],
 }
 
@@ -79,12 +88,14 @@ def load() -> C.CDLL:
     lib.cusoap_last_error.restype = C.c_char_p
     lib.cusoap_profile_enable.argtypes = [i32]
     lib.cusoap_profile_enable.restype = None
+    lib.cusoap_mlp_param_count.argtypes = [i32, i32]
+    lib.cusoap_mlp_param_count.restype = i64
     _lib = lib
     return lib
 
 
 def exported_symbols():
-    return ["cusoap_last_error", "cusoap_profile_enable", *_SIGS.keys()]
+    return ["cusoap_last_error", "cusoap_profile_enable", "cusoap_mlp_param_count", *_SIGS.keys()]
 
 
 def profile_enable(on: bool) -> None:

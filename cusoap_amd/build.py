@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcusoap_amd.so")
-SOURCES = ["capi.hip", "neighbor.hip", "coeffs.hip", "spectrum.hip", "jacobian.hip", "adjoint.hip"]
+SOURCES = ["capi.hip", "neighbor.hip", "coeffs.hip", "spectrum.hip", "jacobian.hip", "adjoint.hip", "surrogate.hip"]
 ARCH = os.environ.get("CUSOAP_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops: never emit packed FP32 VALU ops (v_pk_fma_f32 & co).

@@ -103,20 +103,30 @@ template <>
 __device__ __forceinline__ double t_exp2<double>(double v) { return exp(v); }
 
 // GTO adjoint, thread per edge.  TM: type of M and of the contraction.
-// MIXED: M = G (no beta), radial B/Bt per edge in float64 then rounded to TM.
+// RAD: kRadPrim  M has beta folded in; radial vectors are the primitives.
+//      kRadMixed M = G (no beta), radial B/Bt per edge in float64 then rounded to TM.
+//      kRadTable M = G, B and Bt = (1/r) dB/dr by cubic Hermite interpolation of
+//                basis.rad_table (the derivative of the interpolant, so forward and
+//                adjoint describe the same function).
 // The radial vectors of the current degree live in LDS columns [k][thread].
-template <int LMAX, typename TM, bool MIXED>
+constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
+
+template <int LMAX, typename TM, int RAD>
 __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
                                                       const int32_t* __restrict__ atom_sp,
                                                       const double* __restrict__ shifts,
                                                       const double* __restrict__ centers,
                                                       const int32_t* __restrict__ center_attach,
+                                                      const int32_t* __restrict__ center_species,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr, const TM* __restrict__ M,
                                                       double* __restrict__ grad) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     constexpr int NT = 256;
+    constexpr bool MIXED = RAD == kRadMixed;
     const int n = b.n_max, S = b.n_species;
+    int cslot = S;
+    if (RAD == kRadTable && center_species[blockIdx.x] >= 0) cslot = center_species[blockIdx.x];
     extern __shared__ __align__(16) unsigned char smem[];
     double* qd = (double*)smem;                             // [n][NT]   (MIXED) primitives
     TM* rad0 = (TM*)(qd + (MIXED ? n * NT : 0));            // [n][NT]
@@ -147,8 +157,46 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
         const TM* Mrow = Ms + (size_t)atom_sp[a] * L2 * n;
         const TM trx = (TM)rx, tryy = (TM)ry, trz = (TM)rz;
         TM gx = TM(0), gy = TM(0), gz = TM(0);
+        // table interpolation weights: value (v*) and (1/r) d/dr (c*)
+        const float* tb = nullptr;
+        TM v00 = TM(0), v10 = TM(0), v01 = TM(0), v11 = TM(0), c00 = TM(0), c10 = TM(0), c01 = TM(0), c11 = TM(0);
+        if (RAD == kRadTable) {
+            const int G = b.rad_intervals;
+            const double hd = b.rad_rmax / G, xr = r / hd;
+            int i = (int)xr;
+            i = i < G - 1 ? i : G - 1;
+            const double t = xr - i, t2 = t * t, t3 = t2 * t;
+            v00 = (TM)(2.0 * t3 - 3.0 * t2 + 1.0);
+            v01 = (TM)(3.0 * t2 - 2.0 * t3);
+            v10 = (TM)(hd * (t3 - 2.0 * t2 + t));
+            v11 = (TM)(hd * (t3 - t2));
+            if (i == 0) {
+                // the node slope at r = 0 is zero, so P'(r)/r is a polynomial on the first interval
+                c00 = (TM)(6.0 * (t - 1.0) / (hd * hd));
+                c01 = -c00;
+                c11 = (TM)((3.0 * t - 2.0) / hd);
+            } else {
+                const double ir = 1.0 / r;
+                c00 = (TM)((6.0 * t2 - 6.0 * t) / hd * ir);
+                c10 = (TM)((3.0 * t2 - 4.0 * t + 1.0) * ir);
+                c01 = (TM)((6.0 * t - 6.0 * t2) / hd * ir);
+                c11 = (TM)((3.0 * t2 - 2.0 * t) * ir);
+            }
+            tb = b.rad_table + ((size_t)(cslot * S + atom_sp[a]) * (G + 1) + i) * (L1 * 2 * n);
+        }
         auto on_l = [&](int l) {
-            if (MIXED) {
+            if (RAD == kRadTable) {
+                const float* p0 = tb + l * 2 * n;
+                const float* p1 = p0 + L1 * 2 * n;
+                const TM tw = (TM)w, tdwr = (TM)dwr;
+                for (int nn = 0; nn < n; ++nn) {
+                    const TM B0 = (TM)p0[nn], D0 = (TM)p0[n + nn], B1 = (TM)p1[nn], D1 = (TM)p1[n + nn];
+                    const TM B = v00 * B0 + v10 * D0 + v01 * B1 + v11 * D1;
+                    const TM Bh = c00 * B0 + c10 * D0 + c01 * B1 + c11 * D1;
+                    r0c[nn * NT] = tw * B;
+                    r1c[nn * NT] = tw * Bh + tdwr * B;
+                }
+            } else if (MIXED) {
                 for (int k = 0; k < n; ++k) qc[k * NT] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);
                 for (int nn = 0; nn < n; ++nn) {
                     const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
@@ -330,19 +378,20 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
     }
 }
 
-template <int LMAX, typename TM, bool MIXED>
+template <int LMAX, typename TM, int RAD>
 int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
                    const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
-                   const int64_t* offsets, const int32_t* nbr, const void* M, double* grad, hipStream_t st) {
+                   const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
+                   double* grad, hipStream_t st) {
     const size_t smem = sizeof(TM) * ((size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1) + 2 * 256 * (size_t)b.n_max) +
-                        (MIXED ? sizeof(double) * 256 * (size_t)b.n_max : 0);
+                        (RAD == kRadMixed ? sizeof(double) * 256 * (size_t)b.n_max : 0);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: coefficient block too large for LDS");
-    auto kern = vjp_gto_kernel<LMAX, TM, MIXED>;
+    auto kern = vjp_gto_kernel<LMAX, TM, RAD>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_edges", st);
-    kern<<<n_centers, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr,
-                                       (const TM*)M, grad);
+    kern<<<n_centers, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, center_species,
+                                       offsets, nbr, (const TM*)M, grad);
     prof_end("vjp_edges", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -351,17 +400,19 @@ int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, cons
 template <int LMAX>
 int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
                      const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
-                     const int64_t* offsets, const int32_t* nbr, const void* M, double* grad, hipStream_t st) {
-    if (b.rbf == CUSOAP_RBF_GTO) {
-        if (prec == CUSOAP_PREC_F64)
-            return launch_vjp_gto<LMAX, double, false>(b, pos, n_atoms, atom_sp, shifts, centers, n_centers,
-                                                       center_attach, offsets, nbr, M, grad, st);
-        if (prec == CUSOAP_PREC_F32)
-            return launch_vjp_gto<LMAX, float, false>(b, pos, n_atoms, atom_sp, shifts, centers, n_centers,
-                                                      center_attach, offsets, nbr, M, grad, st);
-        return launch_vjp_gto<LMAX, float, true>(b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach,
-                                                 offsets, nbr, M, grad, st);
+                     const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
+                     double* grad, hipStream_t st) {
+#define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, st
+    if (b.rad_table) {
+        if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable>(CUSOAP_VJP_ARGS);
+        return launch_vjp_gto<LMAX, float, kRadTable>(CUSOAP_VJP_ARGS);
     }
+    if (b.rbf == CUSOAP_RBF_GTO) {
+        if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadPrim>(CUSOAP_VJP_ARGS);
+        if (prec == CUSOAP_PREC_F32) return launch_vjp_gto<LMAX, float, kRadPrim>(CUSOAP_VJP_ARGS);
+        return launch_vjp_gto<LMAX, float, kRadMixed>(CUSOAP_VJP_ARGS);
+    }
+#undef CUSOAP_VJP_ARGS
     const int L2 = (LMAX + 1) * (LMAX + 1);
     const size_t smem = sizeof(double) * ((size_t)b.n_species * b.n_max * L2 + 2 * 64 * (size_t)b.n_max * (LMAX + 1));
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: LDS tile too large");
@@ -402,7 +453,7 @@ size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) {
 
 int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
              const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
-             const int64_t* offsets, const int32_t* nbr, const void* coeffs, int coeffs_dtype,
+             const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs, int coeffs_dtype,
              const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast, int n_feat, void* ws,
              size_t ws_bytes, double* grad, hipStream_t st) {
     (void)n_img;
@@ -413,8 +464,10 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
     if (ws_bytes < vjp_ws_bytes(b, n_centers)) return fail(CUSOAP_ERR_WORKSPACE, "vjp: workspace too small");
     CUSOAP_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * (size_t)n_atoms, st));
     if (n_centers == 0) return CUSOAP_OK;
-    if (b->rbf != CUSOAP_RBF_GTO) prec = CUSOAP_PREC_F64;   // the polynomial path is float64
-    const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED) ? 1 : 0;
+    if (b->rad_table && (!center_species || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
+        return fail(CUSOAP_ERR_ARG, "vjp: radial table needs center_species and a valid grid");
+    if (b->rbf != CUSOAP_RBF_GTO && !b->rad_table) prec = CUSOAP_PREC_F64;   // the polynomial path is float64
+    const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED && !b->rad_table) ? 1 : 0;
     int rc;
 #define CUSOAP_ADJ(TC, TG, TM) \
     launch_adjoint<TC, TG, TM>(*b, mode, coeffs, pair_feature, gd, gd_broadcast, n_feat, fold, n_centers, ws, st)
@@ -429,7 +482,7 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
 #undef CUSOAP_ADJ
     if (rc) return rc;
 #define CUSOAP_CASE(L) \
-    case L: return launch_vjp_edges<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, offsets, nbr, ws, grad, st);
+    case L: return launch_vjp_edges<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, grad, st);
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)

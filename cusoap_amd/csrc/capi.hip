@@ -62,9 +62,23 @@ int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const in
                         const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, int, hipStream_t);
 size_t vjp_ws_bytes(const cusoap_basis_t*, int);
 int vjp_impl(const cusoap_basis_t*, int, int, const double*, int, const int32_t*, const double*, int, const double*,
-             int, const int32_t*, const int64_t*, const int32_t*, const void*, int, const int32_t*, const void*, int,
-             int, int, void*, size_t, double*, hipStream_t);
+             int, const int32_t*, const int32_t*, const int64_t*, const int32_t*, const void*, int, const int32_t*,
+             const void*, int, int, int, void*, size_t, double*, hipStream_t);
 int lowdin_impl(const double*, int, int, double, double*, hipStream_t);
+int64_t mlp_param_count(int, int);
+int radial_exact_impl(const cusoap_basis_t*, const double*, int, double*, double*, hipStream_t);
+int mlp_eval_impl(const float*, int, int, int, const float*, int, float*, float*, hipStream_t);
+size_t mlp_train_ws_bytes(int, int, int);
+int mlp_loss_grad_impl(const float*, int, int, int, const float*, const float*, const float*, int, int64_t,
+                       const float*, const float*, int, uint32_t, int64_t, void*, size_t, float*, float*,
+                       hipStream_t);
+int adam_impl(float*, const float*, float*, float*, int64_t, float, float, float, float, int64_t, hipStream_t);
+size_t mlp_refit_ws_bytes(int, int, int);
+int mlp_refit_impl(float*, int, int, int, const float*, const float*, const float*, int, int64_t, const float*,
+                   double, void*, size_t, hipStream_t);
+size_t radial_table_ws_bytes(const cusoap_basis_t*, int);
+int radial_table_impl(const cusoap_basis_t*, const float*, int, const int32_t*, const float*, double, int, double,
+                      void*, size_t, float*, hipStream_t);
 
 }  // namespace cusoap
 
@@ -170,17 +184,77 @@ int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, si
 
 int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision, const double* positions,
                int32_t n_atoms, const int32_t* atom_species, const double* shifts, int32_t n_img,
-               const double* centers, int32_t n_centers, const int32_t* center_attach, const int64_t* offsets,
-               const int32_t* nbr, const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
-               const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat, void* ws,
-               size_t ws_bytes, double* grad_pos, void* stream) {
+               const double* centers, int32_t n_centers, const int32_t* center_attach,
+               const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs,
+               int32_t coeffs_dtype, const int32_t* pair_feature, const void* grad_desc, int32_t gd_dtype,
+               int32_t gd_broadcast, int32_t n_feat, void* ws, size_t ws_bytes, double* grad_pos, void* stream) {
     return vjp_impl(basis, mode, precision, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
-                    center_attach, offsets, nbr, coeffs, coeffs_dtype, pair_feature, grad_desc, gd_dtype,
-                    gd_broadcast, n_feat, ws, ws_bytes, grad_pos, (hipStream_t)stream);
+                    center_attach, center_species, offsets, nbr, coeffs, coeffs_dtype, pair_feature, grad_desc,
+                    gd_dtype, gd_broadcast, n_feat, ws, ws_bytes, grad_pos, (hipStream_t)stream);
 }
 
 int cusoap_lowdin_invsqrt_f64(const double* S, int32_t batch, int32_t n, double eps, double* out, void* stream) {
     return lowdin_impl(S, batch, n, eps, out, (hipStream_t)stream);
+}
+
+int64_t cusoap_mlp_param_count(int32_t n_hidden, int32_t n_out) { return mlp_param_count(n_hidden, n_out); }
+
+int cusoap_radial_exact(const cusoap_basis_t* basis, const double* r, int32_t n_pts, double* B, double* dBdr,
+                        void* stream) {
+    return radial_exact_impl(basis, r, n_pts, B, dBdr, (hipStream_t)stream);
+}
+
+int cusoap_mlp_eval(const float* params, int32_t n_pairs, int32_t n_hidden, int32_t n_out, const float* x,
+                    int32_t n_pts, float* y, float* dydx, void* stream) {
+    return mlp_eval_impl(params, n_pairs, n_hidden, n_out, x, n_pts, y, dydx, (hipStream_t)stream);
+}
+
+int cusoap_mlp_train_workspace_size(int32_t n_pairs, int32_t n_hidden, int32_t n_out, int32_t batch, size_t* bytes) {
+    if (!bytes) return fail(CUSOAP_ERR_ARG, "mlp_train_workspace_size: null argument");
+    if (mlp_param_count(n_hidden, n_out) < 0) return fail(CUSOAP_ERR_UNSUPPORTED, "mlp: unsupported network shape");
+    *bytes = mlp_train_ws_bytes(n_pairs, n_out, batch);
+    return CUSOAP_OK;
+}
+
+int cusoap_mlp_loss_grad(const float* params, int32_t n_pairs, int32_t n_hidden, int32_t n_out, const float* x,
+                         const float* t, const float* dt, int32_t n_samples, int64_t target_pair_stride,
+                         const float* w_val, const float* w_der, int32_t batch, uint32_t seed, int64_t step, void* ws,
+                         size_t ws_bytes, float* grad, float* loss, void* stream) {
+    return mlp_loss_grad_impl(params, n_pairs, n_hidden, n_out, x, t, dt, n_samples, target_pair_stride, w_val, w_der,
+                              batch, seed, step, ws, ws_bytes, grad, loss, (hipStream_t)stream);
+}
+
+int cusoap_adam_step(float* params, const float* grad, float* m, float* v, int64_t n, float lr, float beta1,
+                     float beta2, float eps, int64_t step, void* stream) {
+    return adam_impl(params, grad, m, v, n, lr, beta1, beta2, eps, step, (hipStream_t)stream);
+}
+
+int cusoap_mlp_refit_workspace_size(int32_t n_pairs, int32_t n_hidden, int32_t n_out, int32_t n_samples,
+                                    size_t* bytes) {
+    if (!bytes) return fail(CUSOAP_ERR_ARG, "mlp_refit_workspace_size: null argument");
+    if (mlp_param_count(n_hidden, n_out) < 0) return fail(CUSOAP_ERR_UNSUPPORTED, "mlp: unsupported network shape");
+    *bytes = mlp_refit_ws_bytes(n_pairs, n_out, n_samples);
+    return CUSOAP_OK;
+}
+
+int cusoap_mlp_refit_output(float* params, int32_t n_pairs, int32_t n_hidden, int32_t n_out, const float* x,
+                            const float* t, const float* dt, int32_t n_samples, int64_t target_pair_stride,
+                            const float* w_der, double ridge, void* ws, size_t ws_bytes, void* stream) {
+    return mlp_refit_impl(params, n_pairs, n_hidden, n_out, x, t, dt, n_samples, target_pair_stride, w_der, ridge, ws,
+                          ws_bytes, (hipStream_t)stream);
+}
+
+int cusoap_radial_table_workspace_size(const cusoap_basis_t* basis, int32_t n_intervals, size_t* bytes) {
+    if (!basis || !bytes) return fail(CUSOAP_ERR_ARG, "radial_table_workspace_size: null argument");
+    *bytes = radial_table_ws_bytes(basis, n_intervals);
+    return CUSOAP_OK;
+}
+
+int cusoap_radial_table(const cusoap_basis_t* basis, const float* params, int32_t n_hidden, const int32_t* use_pair,
+                        const float* scale, double r_fit, int32_t n_intervals, double r_max, void* ws,
+                        size_t ws_bytes, float* table, void* stream) {
+    return radial_table_impl(basis, params, n_hidden, use_pair, scale, r_fit, n_intervals, r_max, ws, ws_bytes, table,
+                             (hipStream_t)stream);
 }
 
 }  // extern "C"

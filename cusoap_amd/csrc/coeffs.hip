@@ -27,6 +27,9 @@
 //          float once; harmonics and accumulation in float.
 // Polynomial basis: the quadrature radial integral is evaluated in double and
 // already orthonormal (no beta).
+// Radial table (basis.rad_table, the learned radial model): B_nl(r) by cubic
+// Hermite interpolation of the tabulated values and slopes, already orthonormal;
+// float accumulation except for F64.
 //
 // create_path: edges with |r| <= 1e-12 are skipped and the l=0 self term is
 // added for index centers, as the reference does; otherwise every edge counts.
@@ -36,6 +39,7 @@ namespace cusoap {
 
 namespace {
 
+constexpr int kRbfTable = 2;     // kernel variant: tabulated radial model
 constexpr int kTE = 64;          // edges per tile
 constexpr int kTEP = kTE + 1;    // padded LDS row length
 constexpr int kThreads = 256;
@@ -102,6 +106,7 @@ __global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const
     const int npair = (n + 1) / 2;
     const int nslot_all = npair * NG;                 // (n pair, lm group) slots per species
     const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
+    const int cslot = (RBF == kRbfTable && self_sp[c] >= 0) ? self_sp[c] : S;   // table slot of the center
 
     for (int s0 = 0; s0 < S; s0 += 4) {
         const int s1 = (s0 + 4 < S) ? s0 + 4 : S;
@@ -155,6 +160,7 @@ __global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const
                 // ---- A2: harmonics (item per edge) and radial factors
                 int n_rad;
                 if (RBF == CUSOAP_RBF_POLY) n_rad = nt * n;
+                else if (RBF == kRbfTable) n_rad = nt * L1;
                 else if (kBetaPerCenter) n_rad = nt * NL;
                 else n_rad = nt * L1;
                 for (int item = tid; item < nt + n_rad; item += kThreads) {
@@ -167,7 +173,26 @@ __global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const
                         continue;
                     }
                     const int it = item - nt;
-                    if (RBF == CUSOAP_RBF_GTO && kBetaPerCenter) {
+                    if (RBF == kRbfTable) {
+                        const int ll = it / nt, e = it - ll * nt;
+                        if (spe[e] < 0) continue;
+                        const int G = b.rad_intervals;
+                        const double xr = geo[e * 8 + 4] * ((double)G / b.rad_rmax);
+                        int i = (int)xr;
+                        i = i < G - 1 ? i : G - 1;
+                        const TA t = (TA)(xr - i), hh = (TA)(b.rad_rmax / G);
+                        const TA t2 = t * t, t3 = t2 * t;
+                        const TA h00 = TA(2) * t3 - TA(3) * t2 + TA(1), h01 = TA(3) * t2 - TA(2) * t3;
+                        const TA h10 = hh * (t3 - TA(2) * t2 + t), h11 = hh * (t3 - t2);
+                        const int NS = L1 * 2 * n;
+                        const float* p0 = b.rad_table + ((size_t)(cslot * S + spe[e]) * (G + 1) + i) * NS + ll * 2 * n;
+                        const float* p1 = p0 + NS;
+                        const TA w = (TA)geo[e * 8 + 5];
+                        for (int nn = 0; nn < n; ++nn) {
+                            const TA B = h00 * (TA)p0[nn] + h10 * (TA)p0[n + nn] + h01 * (TA)p1[nn] + h11 * (TA)p1[n + nn];
+                            Rs[(nn * L1 + ll) * kTEP + e] = w * B;
+                        }
+                    } else if (RBF == CUSOAP_RBF_GTO && kBetaPerCenter) {
                         const int t = it / nt, e = it - t * nt;
                         if (spe[e] < 0) continue;
                         const int k = t / L1, ll = t - k * L1;
@@ -329,6 +354,10 @@ int dispatch_prec(const cusoap_basis_t& b, int prec, const double* pos, int n_at
                   const double* shifts, const double* centers, int n_centers, const int32_t* self_sp,
                   const int64_t* offsets, const int32_t* nbr, int create_path, void* out, hipStream_t st) {
 #define CUSOAP_COEFF_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st
+    if (b.rad_table) {
+        if (prec == CUSOAP_PREC_F64) return launch_coeff<LMAX, CUSOAP_PREC_F64, kRbfTable>(CUSOAP_COEFF_ARGS);
+        return launch_coeff<LMAX, CUSOAP_PREC_MIXED, kRbfTable>(CUSOAP_COEFF_ARGS);
+    }
     if (b.rbf == CUSOAP_RBF_GTO) {
         if (prec == CUSOAP_PREC_F64) return launch_coeff<LMAX, CUSOAP_PREC_F64, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
         if (prec == CUSOAP_PREC_F32) return launch_coeff<LMAX, CUSOAP_PREC_F32, CUSOAP_RBF_GTO>(CUSOAP_COEFF_ARGS);
@@ -351,6 +380,8 @@ int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int 
         return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: n_max must be 1..16 and l_max 0..9");
     if (prec < 0 || prec > 2) return fail(CUSOAP_ERR_ARG, "coefficients: unknown precision");
     if (create_path && !self_sp) return fail(CUSOAP_ERR_ARG, "coefficients: create path needs center_self_species");
+    if (b->rad_table && (!self_sp || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
+        return fail(CUSOAP_ERR_ARG, "coefficients: radial table needs center_self_species and a valid grid");
     if (n_centers == 0) return CUSOAP_OK;
 #define CUSOAP_CASE(L) \
     case L: return dispatch_prec<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st);

@@ -174,14 +174,21 @@ class SOAP:
 
     Extra keyword ``precision`` selects the arithmetic of the coefficient stage:
     "float64" (default, what the reference computes internally), "mixed"
-    (float32 harmonics, float64 radial functions and accumulation) or
+    (float32 harmonics and accumulation, float64 radial functions) or
     "float32".  Output tensors are ``dtype`` as in the reference.
+
+    Extra keyword ``radial``: "exact" (default) or "learned".  "learned" trains
+    (or loads, ``surrogate=<checkpoint path>``) per-species-pair networks for the
+    GTO radial functions on the device.  The coefficient and adjoint kernels then
+    read a table of the gated model (cusoap_amd.surrogate).  ``surrogate`` may
+    also be a RadialSurrogate or a dict of its constructor and train() options.
+    The dense Jacobian always uses the exact radial functions.
     """
 
     def __init__(self, r_cut=None, n_max=None, l_max=None, sigma=1.0, rbf="gto", weighting=None,
                  crossover=True, average="off", species=None, periodic=False, sparse=False,
                  dtype="float32", device=None, quad_n=100, max_num_neighbors=None,
-                 precision="float64", _betas=None):
+                 precision="float64", radial="exact", surrogate=None, table_intervals=1024, _betas=None):
         if n_max is None or l_max is None:
             raise ValueError("n_max and l_max must be provided.")
         if species is None or len(species) == 0:
@@ -245,7 +252,37 @@ class SOAP:
         self.n_features = int(self.get_number_of_features())
         self._build_feature_tables()
         self._basis_struct = None
+        self._basis_learned = None
         self._last_nl_backend = "hip-cell-list"
+        if radial not in ("exact", "learned"):
+            raise ValueError("radial must be 'exact' or 'learned'")
+        self.radial = radial
+        self.surrogate = None
+        self._rad_table = None
+        if radial == "learned":
+            self._init_learned_radial(surrogate, int(table_intervals))
+
+    def _init_learned_radial(self, surrogate, n_intervals: int):
+        """Train or load the radial networks, gate them, tabulate the radial model."""
+        from .surrogate import RadialSurrogate, radial_table
+        self._require_gpu()
+        if self._rbf != "gto":
+            raise NotImplementedError("radial='learned' covers the GTO basis")
+        if isinstance(surrogate, RadialSurrogate):
+            model = surrogate
+        else:
+            opts = dict(surrogate) if isinstance(surrogate, dict) else {}
+            ckpt = surrogate if isinstance(surrogate, str) else opts.pop("checkpoint", None)
+            init_keys = ("r_fit", "n_train", "n_heldout", "seed", "deriv_weight", "tol", "tol_slope")
+            model = RadialSurrogate(self, **{k: opts.pop(k) for k in init_keys if k in opts})
+            if ckpt:
+                model.load(ckpt)
+            else:
+                model.train(**opts)
+        model.gate()
+        self.surrogate = model
+        self._rad_intervals = n_intervals
+        self._rad_table = radial_table(self, n_intervals, model=model)
 
     # ---- basis (soap_generator.py:690-819) ---------------------------------
     def _init_gto_basis(self, betas=None):
@@ -405,6 +442,19 @@ class SOAP:
         return t.to(self.device, non_blocking=False)
 
     def _basis(self) -> "_lib.Basis":
+        """Basis tables for the kernels, with the radial table when radial='learned'."""
+        b = self._basis_exact()
+        if self._rad_table is None:
+            return b
+        if self._basis_learned is None:
+            bl = _lib.Basis.from_buffer_copy(b)
+            bl.rad_table = self._rad_table.data_ptr()
+            bl.rad_intervals = self._rad_intervals
+            bl.rad_rmax = self._cutoff
+            self._basis_learned = bl
+        return self._basis_learned
+
+    def _basis_exact(self) -> "_lib.Basis":
         if self._basis_struct is not None:
             return self._basis_struct
         self._require_gpu()
@@ -532,10 +582,11 @@ class SOAP:
     def _coeff_dtype(self, precision: str) -> torch.dtype:
         return torch.float64 if precision == "float64" else torch.float32
 
-    def _coefficients(self, x: _Ctx, nl: _NL, create_path: bool, precision: Optional[str] = None) -> torch.Tensor:
+    def _coefficients(self, x: _Ctx, nl: _NL, create_path: bool, precision: Optional[str] = None,
+                      exact: bool = False) -> torch.Tensor:
         precision = precision or self.precision
         lib = _lib.load()
-        b = self._basis()
+        b = self._basis_exact() if exact else self._basis()
         L2 = (self._l_max + 1) ** 2
         out = torch.empty((x.n_centers, self.n_species, self._n_max, L2), dtype=self._coeff_dtype(precision),
                           device=self.device)
@@ -656,7 +707,7 @@ class SOAP:
     def _dense_jacobian(self, system, centers, include, exclude, attach, return_descriptor):
         x = self._context(system, centers)
         lib = _lib.load()
-        b = self._basis()
+        b = self._basis_exact()
         idx = self._included(x.n_atoms, include, exclude)
         n_inc = len(idx)
         inv = -np.ones(x.n_atoms, dtype=np.int32)
@@ -666,7 +717,7 @@ class SOAP:
         nl_keep = self._neighbors(x, drop_self=False)
         # the dense Jacobian kernels take float64 coefficients
         prec = "mixed" if self.precision == "mixed" else "float64"
-        coeffs = self._coefficients(x, nl_keep, create_path=not closed, precision=prec)
+        coeffs = self._coefficients(x, nl_keep, create_path=not closed, precision=prec, exact=True)
         if coeffs.dtype != torch.float64:
             coeffs = coeffs.double()
         mode = _lib.MODE_CC if self.average == "cc" else _lib.MODE_PS
@@ -811,7 +862,8 @@ class SOAP:
         _lib.check(lib.cusoap_vjp(
             C.byref(b), mode, _PRECISIONS[self.precision], x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(),
             x.shifts.data_ptr(), x.n_img, x.centers.data_ptr(), x.n_centers,
-            att.data_ptr() if att is not None else None, nl_keep.offsets.data_ptr(), nl_keep.nbr.data_ptr(),
+            att.data_ptr() if att is not None else None, x.self_sp.data_ptr(), nl_keep.offsets.data_ptr(),
+            nl_keep.nbr.data_ptr(),
             coeffs.data_ptr(), _lib.DT_F64 if coeffs.dtype == torch.float64 else _lib.DT_F32,
             self._keep["pf"].data_ptr(), g.data_ptr(), _lib.DT_F64 if g.dtype == torch.float64 else _lib.DT_F32,
             bcast, n_feat, ws.data_ptr(), ws.numel(), grad.data_ptr(), self._stream()), "vjp")

@@ -68,6 +68,15 @@ typedef struct cusoap_basis {
     const double* poly_B0;      /* (n_max, l_max+1)  r -> 0 limit of B              */
     const double* poly_Bh0;     /* (n_max, l_max+1)  r -> 0 limit of Bhat           */
     const double* self_l0;      /* (n_max)           l=0 self coefficient (incl. w0, Y00) */
+    /* Tabulated radial model (the learned radial surrogate, see cusoap_radial_table),
+     * float32 device array or NULL for the exact radial functions.  Layout
+     * ((S+1) S slots, rad_intervals+1 nodes, l_max+1, 2, n_max): B_nl(r_i) then
+     * dB_nl/dr(r_i) at r_i = i rad_rmax / rad_intervals; slot = center_slot * S +
+     * neighbour species, center_slot = the center atom's species, or S for a
+     * center that is not an atom.  Kernels interpolate cubic Hermite in r. */
+    const float* rad_table;
+    int32_t rad_intervals;
+    double rad_rmax;
 } cusoap_basis_t;
 
 /* Cell-list geometry of one neighbour search. */
@@ -129,7 +138,9 @@ int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_
  * functions (with weighting), accumulation and the Loewdin contraction.
  * create_path != 0: edges with |r| <= 1e-12 are skipped and self_l0 is added
  * for centers with center_self_species >= 0 (index centers); otherwise every
- * edge is kept (the closed-form derivative path).
+ * edge is kept (the closed-form derivative path).  center_self_species (the
+ * species of the center atom, -1 for a coordinate center) also selects the
+ * radial-table slot when basis->rad_table is set; it is then required.
  * coeffs: (n_centers, n_species, n_max, (l_max+1)^2), float64 for PREC_F64,
  * float32 for PREC_MIXED / PREC_F32. */
 int cusoap_coefficients(const cusoap_basis_t* basis, int32_t precision,
@@ -183,13 +194,15 @@ int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
  * centers.  precision: CUSOAP_PREC_* of the edge pass (the polynomial basis
  * always runs in float64).  pair_feature: (l_max+1, S n, S n) int32 feature
  * index of each channel pair or -1 (power spectrum only).  center_attach as
- * for the Jacobian.  Forces are accumulated with float64 atomics. */
+ * for the Jacobian.  center_species: (n_centers) species of the center atom or
+ * -1, required when basis->rad_table is set (table slot), else may be NULL.
+ * Forces are accumulated with float64 atomics. */
 int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes);
 int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
                const double* positions, int32_t n_atoms, const int32_t* atom_species,
                const double* shifts, int32_t n_img,
                const double* centers, int32_t n_centers, const int32_t* center_attach,
-               const int64_t* offsets, const int32_t* nbr,
+               const int32_t* center_species, const int64_t* offsets, const int32_t* nbr,
                const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
                const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
                void* ws, size_t ws_bytes, double* grad_pos, void* stream);
@@ -201,6 +214,62 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
  * at eps before the inverse square root.  n <= 32. */
 int cusoap_lowdin_invsqrt_f64(const double* S, int32_t batch, int32_t n, double eps,
                               double* out, void* stream);
+
+/* ---- learned radial surrogate (no reference counterpart; it stands in for the
+ *      exact radial integral of _coefficients_gto and derivatives_analytical_ps,
+ *      soap_generator.py:1358-1391, 2389-2405, where its held-out error allows) --
+ * One network per species pair (center species, neighbour species), float32:
+ * x in [-1, 1] -> 64 tanh -> 64 tanh -> n_out linear.  Parameters per pair, in
+ * this order: W1[64] b1[64] W2[64][64] b2[64] W3[n_out][64] b3[n_out]; pairs are
+ * contiguous.  n_hidden must be 64 and n_out <= 128. */
+/* Number of float parameters of one network, or -1 for an unsupported shape. */
+int64_t cusoap_mlp_param_count(int32_t n_hidden, int32_t n_out);
+/* Exact float64 orthonormal GTO radial functions at n_pts radii:
+ * B[p][l][n] = sum_k beta_l[n][k] pref_kl exp(-gamma_kl r_p^2) and dBdr = dB/dr. */
+int cusoap_radial_exact(const cusoap_basis_t* basis, const double* r, int32_t n_pts,
+                        double* B, double* dBdr, void* stream);
+/* Network outputs and input derivatives of every pair at n_pts inputs x:
+ * y, dydx (n_pairs, n_pts, n_out) float32. */
+int cusoap_mlp_eval(const float* params, int32_t n_pairs, int32_t n_hidden, int32_t n_out,
+                    const float* x, int32_t n_pts, float* y, float* dydx, void* stream);
+/* Loss and gradient over a minibatch of `batch` samples drawn (with replacement,
+ * by a hash of seed, step and position) from n_samples training points:
+ * L = 1/(n_out batch) sum [w_val_o (y - t)^2 + w_der_o (dy/dx - dt)^2].
+ * x (n_samples); t, dt (n_pairs, ...) with target_pair_stride elements between
+ * pairs (0 = shared targets), each (n_samples, n_out).  grad (n_pairs, P),
+ * loss (n_pairs) or NULL.  Deterministic (fixed-order reduction). */
+int cusoap_mlp_train_workspace_size(int32_t n_pairs, int32_t n_hidden, int32_t n_out, int32_t batch,
+                                    size_t* bytes);
+int cusoap_mlp_loss_grad(const float* params, int32_t n_pairs, int32_t n_hidden, int32_t n_out,
+                         const float* x, const float* t, const float* dt, int32_t n_samples,
+                         int64_t target_pair_stride, const float* w_val, const float* w_der,
+                         int32_t batch, uint32_t seed, int64_t step, void* ws, size_t ws_bytes,
+                         float* grad, float* loss, void* stream);
+/* Adam (Kingma & Ba 2015) with bias correction, step counted from 1. */
+int cusoap_adam_step(float* params, const float* grad, float* m, float* v, int64_t n, float lr,
+                     float beta1, float beta2, float eps, int64_t step, void* stream);
+/* Exact least-squares refit of the output layer (W3, b3) of every pair for the
+ * current hidden layers, minimising the same loss over all n_samples points
+ * (float64 normal equations, Cholesky; ridge scales trace(A)/65 on the diagonal). */
+int cusoap_mlp_refit_workspace_size(int32_t n_pairs, int32_t n_hidden, int32_t n_out, int32_t n_samples,
+                                    size_t* bytes);
+int cusoap_mlp_refit_output(float* params, int32_t n_pairs, int32_t n_hidden, int32_t n_out,
+                            const float* x, const float* t, const float* dt, int32_t n_samples,
+                            int64_t target_pair_stride, const float* w_der, double ridge,
+                            void* ws, size_t ws_bytes, void* stream);
+/* Radial table for basis->rad_table (layout there) on n_intervals intervals of
+ * [0, r_max].  Node values come from the network of pair p = center species * S +
+ * neighbour species where use_pair[p] != 0 and 0 < r_i <= r_fit, and from the
+ * exact float64 function elsewhere, including r_i = 0 (and for every slot when
+ * params is NULL).  The network models g_nl = u^l B_nl, u = r / r_fit (input
+ * x = 2u - 1, outputs l-major j = l n_max + n, g = scale[p][j] y, dg/dr =
+ * scale[p][j] (2 / r_fit) dy/dx); the table holds B = g / u^l and dB/dr =
+ * g'/u^l - l g / (r_fit u^(l+1)).  dB/dr is 0 at r = 0. */
+int cusoap_radial_table_workspace_size(const cusoap_basis_t* basis, int32_t n_intervals, size_t* bytes);
+int cusoap_radial_table(const cusoap_basis_t* basis, const float* params, int32_t n_hidden,
+                        const int32_t* use_pair, const float* scale, double r_fit,
+                        int32_t n_intervals, double r_max, void* ws, size_t ws_bytes,
+                        float* table, void* stream);
 
 #ifdef __cplusplus
 }
