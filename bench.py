@@ -44,9 +44,11 @@ def parse_args():
     ap.add_argument("--config", default="h2o_1000", choices=["h2o_1000", "water_box", "hcno_box", "h2o_1m"])
     ap.add_argument("--radial", default="exact", choices=["exact", "learned"],
                     help="radial functions of the large configs: exact, or the learned model (trained first)")
+    ap.add_argument("--spectrum", default="exact", choices=["exact", "fp16", "fp16x3", "bf16", "bf16x3"],
+                    help="power-spectrum contraction of the large configs (matrix-core modes)")
     ap.add_argument("--compare", default="",
                     help="comma list of forms to compare against float64 (large configs): a precision, "
-                         "optionally with ':learned'")
+                         "optionally with ':learned' and/or a spectrum mode, e.g. mixed:learned:bf16x3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-centers", type=int, default=48)
     ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
@@ -208,22 +210,27 @@ def bench_adjoint(args, dev):
     soaps = {}
     train = {}
 
-    def soap_for(prec, radial="exact"):
-        key = (prec, radial)
+    def soap_for(prec, radial="exact", spectrum="exact"):
+        key = (prec, radial, spectrum)
         if key not in soaps:
-            if radial == "learned":
+            learned = [v for k, v in soaps.items() if k[1] == "learned" and k[0] == prec]
+            if radial == "learned" and learned:
+                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, radial="learned",
+                                  surrogate=learned[0].surrogate, spectrum=spectrum, **params)
+            elif radial == "learned":
                 t1 = time.perf_counter()
-                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, radial="learned", **params)
+                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, radial="learned", spectrum=spectrum,
+                                  **params)
                 sur = soaps[key].surrogate
                 train.update({"radial_training_s": round(time.perf_counter() - t1, 2),
                               "radial_gate": [int(u) for u in sur.use_pair],
                               "radial_heldout_value_err": float(sur.errors["value"].max()),
                               "radial_heldout_slope_err": float(sur.errors["slope"].max())})
             else:
-                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, **params)
+                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, spectrum=spectrum, **params)
         return soaps[key]
 
-    soap = soap_for(args.precision, args.radial)
+    soap = soap_for(args.precision, args.radial, args.spectrum)
     n_feat = soap.get_number_of_features()
     w = torch.tensor(np.random.default_rng(12345).normal(size=n_feat), device=dev)
 
@@ -256,19 +263,21 @@ def bench_adjoint(args, dev):
                      {"float64": "f64", "mixed": "f64+f32", "float32": "f32"}[args.precision]),
            "data": "synthetic",
            "config": {"workload": desc_txt, "n_atoms": n_env, "n_features": n_feat, "precision": args.precision,
-                      "radial": args.radial, **train,
+                      "radial": args.radial, "spectrum": args.spectrum, **train,
                       "system_generation_s": round(t_gen, 1), "parallelism": "single GPU"},
            "kernel_ms": {k: round(v, 4) for k, v in kern.items()}}
     if args.compare:
         ref_d = d.double()
         ref_F = F
-        if args.precision != "float64" or args.radial != "exact":
+        if args.precision != "float64" or args.radial != "exact" or args.spectrum != "exact":
             ref_d, ref_F = step(soap_for("float64"))
             ref_d = ref_d.double()
         errs = {}
         for form in [p for p in args.compare.split(",") if p]:
-            prec, _, rad = form.partition(":")
-            dd, FF = step(soap_for(prec, rad or "exact"))
+            parts = form.split(":")
+            rad = "learned" if "learned" in parts[1:] else "exact"
+            spec = next((q for q in parts[1:] if q != "learned"), "exact")
+            dd, FF = step(soap_for(parts[0], rad, spec))
             dd = dd.double()
             sc = ref_d.abs().amax(0).clamp_min(1e-300)
             errs[form] = {"desc_max_col_rel": float(((dd - ref_d).abs().amax(0) / sc).max()),

@@ -15,6 +15,7 @@ W_NONE, W_POLY, W_POW, W_EXP = 0, 1, 2, 3
 PREC_F64, PREC_F32, PREC_MIXED = 0, 1, 2
 DT_F64, DT_F32 = 0, 1
 MODE_PS, MODE_CC = 0, 1
+MFMA_F16, MFMA_BF16 = 0, 1
 
 p = C.c_void_p
 i32 = C.c_int32
@@ -49,6 +50,7 @@ _SIGS = {
     "cusoap_coefficients": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i32, p, p],
     "cusoap_power_spectrum": [C.POINTER(Basis), p, i32, i32, p, i32, p, i32, p],
     "cusoap_cc_projection": [C.POINTER(Basis), p, i32, i32, p, i32, p],
+    "cusoap_power_spectrum_mfma": [C.POINTER(Basis), p, i32, i32, p, i32, i32, i32, p, i32, p],
     "cusoap_jacobian_workspace_size": [C.POINTER(Basis), i32, i64, i32, i32, C.POINTER(sz)],
     "cusoap_jacobian_dense": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i64, p, p, i32,
                               p, i32, p, sz, p, i32, i32, p],

@@ -56,6 +56,8 @@ int coefficients_impl(const cusoap_basis_t*, int, const double*, int, const int3
                       const double*, int, const int32_t*, const int64_t*, const int32_t*, int, void*, hipStream_t);
 int power_spectrum_impl(const cusoap_basis_t*, const void*, int, int, const uint32_t*, int, void*, int, hipStream_t);
 int cc_projection_impl(const cusoap_basis_t*, const void*, int, int, void*, int, hipStream_t);
+int power_spectrum_mfma_impl(const cusoap_basis_t*, const void*, int, int, const int32_t*, int, int, int, void*, int,
+                             hipStream_t);
 size_t jacobian_ws_bytes(const cusoap_basis_t*, int, int64_t, int, int);
 int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const int32_t*, const double*, int,
                         const double*, int, const int32_t*, const int64_t*, const int32_t*, int64_t, const double*,
@@ -195,6 +197,13 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision, con
 
 int cusoap_lowdin_invsqrt_f64(const double* S, int32_t batch, int32_t n, double eps, double* out, void* stream) {
     return lowdin_impl(S, batch, n, eps, out, (hipStream_t)stream);
+}
+
+int cusoap_power_spectrum_mfma(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype,
+                               int32_t n_centers, const int32_t* pair_feature, int32_t n_feat, int32_t kind,
+                               int32_t split, void* out, int32_t out_dtype, void* stream) {
+    return power_spectrum_mfma_impl(basis, coeffs, coeffs_dtype, n_centers, pair_feature, n_feat, kind, split, out,
+                                    out_dtype, (hipStream_t)stream);
 }
 
 int64_t cusoap_mlp_param_count(int32_t n_hidden, int32_t n_out) { return mlp_param_count(n_hidden, n_out); }

@@ -156,6 +156,8 @@ class Weighting:
 # ----------------------------------------------------------------------------
 
 _PRECISIONS = {"float64": _lib.PREC_F64, "mixed": _lib.PREC_MIXED, "float32": _lib.PREC_F32}
+_SPECTRA = {"exact": None, "fp16": (_lib.MFMA_F16, 0), "fp16x3": (_lib.MFMA_F16, 1),
+            "bf16": (_lib.MFMA_BF16, 0), "bf16x3": (_lib.MFMA_BF16, 1)}
 
 
 class _NL:
@@ -183,12 +185,19 @@ class SOAP:
     read a table of the gated model (cusoap_amd.surrogate).  ``surrogate`` may
     also be a RadialSurrogate or a dict of its constructor and train() options.
     The dense Jacobian always uses the exact radial functions.
+
+    Extra keyword ``spectrum``: arithmetic of the power-spectrum contraction.
+    "exact" (default) uses the vector units in the coefficient precision.
+    "fp16" / "bf16" use the matrix cores with fp16 / bf16 operands and float32
+    accumulation.  "fp16x3" / "bf16x3" split each operand into hi + lo and sum
+    three products (cusoap_power_spectrum_mfma).
     """
 
     def __init__(self, r_cut=None, n_max=None, l_max=None, sigma=1.0, rbf="gto", weighting=None,
                  crossover=True, average="off", species=None, periodic=False, sparse=False,
                  dtype="float32", device=None, quad_n=100, max_num_neighbors=None,
-                 precision="float64", radial="exact", surrogate=None, table_intervals=1024, _betas=None):
+                 precision="float64", radial="exact", surrogate=None, table_intervals=1024, spectrum="exact",
+                 _betas=None):
         if n_max is None or l_max is None:
             raise ValueError("n_max and l_max must be provided.")
         if species is None or len(species) == 0:
@@ -256,6 +265,9 @@ class SOAP:
         self._last_nl_backend = "hip-cell-list"
         if radial not in ("exact", "learned"):
             raise ValueError("radial must be 'exact' or 'learned'")
+        if spectrum not in _SPECTRA:
+            raise ValueError(f"spectrum must be one of {sorted(_SPECTRA)}")
+        self.spectrum = spectrum
         self.radial = radial
         self.surrogate = None
         self._rad_table = None
@@ -612,6 +624,12 @@ class SOAP:
             return out
         nf = self._n_power_features()
         out = torch.empty((C_, nf), dtype=self.dtype, device=self.device)
+        if self.spectrum != "exact":
+            kind, split = _SPECTRA[self.spectrum]
+            _lib.check(lib.cusoap_power_spectrum_mfma(C.byref(b), coeffs.data_ptr(), cdt, C_,
+                                                      self._keep["pf"].data_ptr(), nf, kind, split, out.data_ptr(),
+                                                      odt, self._stream()), "power_spectrum_mfma")
+            return out
         _lib.check(lib.cusoap_power_spectrum(C.byref(b), coeffs.data_ptr(), cdt, C_, self._keep["ft"].data_ptr(),
                                              nf, out.data_ptr(), odt, self._stream()), "power_spectrum")
         return out

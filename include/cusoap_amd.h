@@ -49,6 +49,7 @@ enum { CUSOAP_W_NONE = 0, CUSOAP_W_POLY = 1, CUSOAP_W_POW = 2, CUSOAP_W_EXP = 3 
 enum { CUSOAP_PREC_F64 = 0, CUSOAP_PREC_F32 = 1, CUSOAP_PREC_MIXED = 2 };
 enum { CUSOAP_DT_F64 = 0, CUSOAP_DT_F32 = 1 };
 enum { CUSOAP_MODE_PS = 0, CUSOAP_MODE_CC = 1 };
+enum { CUSOAP_MFMA_F16 = 0, CUSOAP_MFMA_BF16 = 1 };
 
 /* Radial basis tables, all device pointers to float64.  Mirrors the state the
  * reference builds in SOAP.__init__ (soap_generator.py:600-634, 690-819). */
@@ -159,6 +160,18 @@ int cusoap_power_spectrum(const cusoap_basis_t* basis, const void* coeffs, int32
                           void* out, int32_t out_dtype, void* stream);
 int cusoap_cc_projection(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype,
                          int32_t n_centers, void* out, int32_t out_dtype, void* stream);
+/* The same power spectrum on the matrix cores: per center and l the Gram matrix
+ * of the (S n) x (2l+1) coefficient block from 16x16x16 MFMA tiles with fp16
+ * (kind = CUSOAP_MFMA_F16) or bf16 (CUSOAP_MFMA_BF16) operands and float32
+ * accumulation; split != 0 carries each operand as hi + lo and sums three
+ * products (hi hi + hi lo + lo hi).  The block is scaled by a power of two before
+ * rounding.  pair_feature: (l_max+1, S n, S n) int32 feature index of each
+ * channel pair or -1, symmetric (as for cusoap_vjp).  No reference counterpart
+ * for the arithmetic; it replaces the einsum of _power_spectrum
+ * (soap_generator.py:1634-1689). */
+int cusoap_power_spectrum_mfma(const cusoap_basis_t* basis, const void* coeffs, int32_t coeffs_dtype,
+                               int32_t n_centers, const int32_t* pair_feature, int32_t n_feat,
+                               int32_t kind, int32_t split, void* out, int32_t out_dtype, void* stream);
 
 /* ---- dense analytical Jacobian (replaces derivatives_analytical_ps and its
  *      Triton kernels _ps_uv_kernel / _ps_jacobian_dense_kernel /
