@@ -3,10 +3,14 @@
 Default workload: the non-periodic (H2O)_1000 cluster (3,000 atoms; O sites on a
 jittered 10x10x10 lattice of 3.1 A, random orientations, seed 0), species H/O,
 GTO basis, r_cut=10.0, sigma=0.5, n_max=7, l_max=3, average="off", float32
-output with the reference's float64 internal arithmetic.  One step = one
-create() of every center plus one derivatives(method="analytical") call, i.e.
-the full dense (3000, 3000, 3, 420) Jacobian, as the reference's serial driver
-does.  Positions are already resident in HBM when the timed region starts.
+output.  One step = one create() of every center plus one
+derivatives(method="analytical") call, i.e. the full dense (3000, 3000, 3, 420)
+Jacobian, as the reference's serial driver does.  Positions are already resident
+in HBM when the timed region starts.  The timed form is --precision (default
+"mixed": float64 radial functions, float32 harmonics, accumulation and
+single-edge Jacobian rows); the plain float64 form (the reference's internal
+arithmetic) is timed next to it and the chosen form's per-feature error against
+it is reported ("plain_float64_form").
 
 Other workloads (--config), each step = create() + the adjoint forces of a
 seeded Gaussian linear readout (descriptor_and_vjp; no dp/dR is formed):
@@ -40,17 +44,21 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="float64", choices=["float64", "mixed", "float32"])
+    ap.add_argument("--precision", default="mixed", choices=["float64", "mixed", "float32"])
+    ap.add_argument("--no-float64-reference", action="store_true",
+                    help="(h2o_1000) skip timing the plain float64 form next to the chosen one")
     ap.add_argument("--config", default="h2o_1000", choices=["h2o_1000", "water_box", "hcno_box", "h2o_1m"])
     ap.add_argument("--radial", default="exact", choices=["exact", "learned"],
                     help="radial functions of the large configs: exact, or the learned model (trained first)")
+    ap.add_argument("--radial-checkpoint", default="", metavar="NPZ",
+                    help="learned radial networks: load this checkpoint if it exists, else train and save it")
     ap.add_argument("--spectrum", default="exact", choices=["exact", "fp16", "fp16x3", "bf16", "bf16x3"],
                     help="power-spectrum contraction of the large configs (matrix-core modes)")
     ap.add_argument("--compare", default="",
                     help="comma list of forms to compare against float64 (large configs): a precision, "
                          "optionally with ':learned' and/or a spectrum mode, e.g. mixed:learned:bf16x3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-centers", type=int, default=48)
+    ap.add_argument("--cpu-centers", type=int, default=200)
     ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GB",
                     help="only run one plain device-to-device copy of GB gigabytes (PMC counter calibration)")
@@ -86,8 +94,8 @@ def cpu_baseline(system, params, n_centers):
 def main():
     args = parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the work would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (centers are independent; "
+              "a spatial split with a halo exchange of forces is the design for several GPUs)", file=sys.stderr)
         sys.exit(2)
     import torch
     from cusoap_amd import SOAP, _lib, systems
@@ -106,37 +114,62 @@ def main():
         return bench_adjoint(args, dev)
     params = dict(r_cut=10.0, n_max=7, l_max=3, sigma=0.5, species=[1, 8], rbf="gto", average="off")
     system = systems.water_cluster_3000(seed=0)
-    soap = SOAP(device=dev, dtype="float32", precision=args.precision, **params)
     sys_dev = {"positions": torch.tensor(system["positions"], device=dev),
                "atomic_numbers": system["atomic_numbers"]}
     n_env = system["positions"].shape[0]
+    names = ("jacobian_dense", "jacobian_edges", "coefficients", "nl_count", "nl_fill", "power_spectrum")
+
+    def run(precision):
+        soap = SOAP(device=dev, dtype="float32", precision=precision, **params)
+
+        def step():
+            d = soap.create(sys_dev)
+            J = soap.derivatives(sys_dev, method="analytical", return_descriptor=False)
+            return d, J
+
+        for _ in range(args.warmup):
+            d, J = step()
+            del d, J
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        for name in names:
+            _lib.profile_get(name)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d, J = step()
+            del d, J
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        _lib.profile_enable(False)
+        kern = {}
+        for name in names:
+            ms, n = _lib.profile_get(name)
+            kern[name] = ms / max(n, 1)
+        return soap, step, dt, kern
+
+    soap, step, dt, kern = run(args.precision)
     n_feat = soap.get_number_of_features()
-
-    def step():
-        d = soap.create(sys_dev)
-        J = soap.derivatives(sys_dev, method="analytical", return_descriptor=False)
-        return d, J
-
-    for _ in range(args.warmup):
-        d, J = step()
-        del d, J
-    torch.cuda.synchronize()
-    _lib.profile_enable(True)
-    for name in ("jacobian_dense", "jacobian_edges", "coefficients", "nl_count", "nl_fill", "power_spectrum"):
-        _lib.profile_get(name)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        d, J = step()
-        del d, J
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    _lib.profile_enable(False)
-    kern = {}
-    for name in ("jacobian_dense", "jacobian_edges", "coefficients", "nl_count", "nl_fill", "power_spectrum"):
-        ms, n = _lib.profile_get(name)
-        kern[name] = ms / max(n, 1)
     ms_step = dt * 1e3 / args.steps
+    ref_form = None
+    if args.precision != "float64" and not args.no_float64_reference:
+        d, J = step()
+        soap64, step64, dt64, kern64 = run("float64")
+        d64, J64 = step64()
+        err_d = float(((d.double() - d64.double()).abs().amax(0) / d64.double().abs().amax(0).clamp_min(1e-300)).max())
+        num = torch.zeros(n_feat, dtype=torch.float64, device=dev)
+        den = torch.zeros(n_feat, dtype=torch.float64, device=dev)
+        for c0 in range(0, n_env, 100):                     # per-feature column max over (center, atom, xyz)
+            a, b = J[c0:c0 + 100].double(), J64[c0:c0 + 100].double()
+            num = torch.maximum(num, (a - b).abs().amax(dim=(0, 1, 2)))
+            den = torch.maximum(den, b.abs().amax(dim=(0, 1, 2)))
+        err_j = float((num / den.clamp_min(1e-300)).max())
+        del d, J, d64, J64
+        ref_form = {"precision": "float64", "value": round(n_env * args.steps / dt64, 1),
+                    "ms_per_step": round(dt64 * 1e3 / args.steps, 3),
+                    "kernel_ms": {k: round(v, 4) for k, v in kern64.items()},
+                    "chosen_form_max_col_rel_err": {"descriptor": err_d, "jacobian": err_j},
+                    "speedup_of_chosen_form": round(dt64 / dt, 3)}
     value = n_env * args.steps / dt
     jac_bytes = float(n_env) * n_env * 3 * n_feat * 4          # algorithmic: every output element stored once
     achieved = jac_bytes / (kern["jacobian_dense"] * 1e-3) / 1e9
@@ -164,6 +197,8 @@ def main():
                      "avg_kernel_ms": round(kern["jacobian_dense"], 4)},
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }
+    if ref_form is not None:
+        out["plain_float64_form"] = ref_form
     if args.check:
         from oracle import soap_oracle as so
         ref = so.OracleSOAP(**params).create(system)
@@ -219,9 +254,14 @@ def bench_adjoint(args, dev):
                                   surrogate=learned[0].surrogate, spectrum=spectrum, **params)
             elif radial == "learned":
                 t1 = time.perf_counter()
+                ckpt = args.radial_checkpoint
+                src = ckpt if ckpt and os.path.exists(ckpt) else None
                 soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, radial="learned", spectrum=spectrum,
-                                  **params)
+                                  surrogate=src, **params)
                 sur = soaps[key].surrogate
+                if ckpt and src is None:
+                    sur.save(ckpt)
+                train["radial_checkpoint_loaded"] = src is not None
                 train.update({"radial_training_s": round(time.perf_counter() - t1, 2),
                               "radial_gate": [int(u) for u in sur.use_pair],
                               "radial_heldout_value_err": float(sur.errors["value"].max()),
