@@ -189,12 +189,24 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
                 const float* p0 = tb + l * 2 * n;
                 const float* p1 = p0 + L1 * 2 * n;
                 const TM tw = (TM)w, tdwr = (TM)dwr;
-                for (int nn = 0; nn < n; ++nn) {
-                    const TM B0 = (TM)p0[nn], D0 = (TM)p0[n + nn], B1 = (TM)p1[nn], D1 = (TM)p1[n + nn];
-                    const TM B = v00 * B0 + v10 * D0 + v01 * B1 + v11 * D1;
-                    const TM Bh = c00 * B0 + c10 * D0 + c01 * B1 + c11 * D1;
+                auto emit = [&](int nn, float b0, float d0, float b1, float d1) {
+                    const TM B = v00 * (TM)b0 + v10 * (TM)d0 + v01 * (TM)b1 + v11 * (TM)d1;
+                    const TM Bh = c00 * (TM)b0 + c10 * (TM)d0 + c01 * (TM)b1 + c11 * (TM)d1;
                     r0c[nn * NT] = tw * B;
                     r1c[nn * NT] = tw * Bh + tdwr * B;
+                };
+                if ((n & 3) == 0) {
+                    // node rows start on 16-byte boundaries: four 16-byte loads per 4 functions
+                    for (int k4 = 0; k4 < n; k4 += 4) {
+                        const float4 b0 = *(const float4*)(p0 + k4), d0 = *(const float4*)(p0 + n + k4);
+                        const float4 b1 = *(const float4*)(p1 + k4), d1 = *(const float4*)(p1 + n + k4);
+                        emit(k4, b0.x, d0.x, b1.x, d1.x);
+                        emit(k4 + 1, b0.y, d0.y, b1.y, d1.y);
+                        emit(k4 + 2, b0.z, d0.z, b1.z, d1.z);
+                        emit(k4 + 3, b0.w, d0.w, b1.w, d1.w);
+                    }
+                } else {
+                    for (int nn = 0; nn < n; ++nn) emit(nn, p0[nn], p0[n + nn], p1[nn], p1[n + nn]);
                 }
             } else if (MIXED) {
                 for (int k = 0; k < n; ++k) qc[k * NT] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);

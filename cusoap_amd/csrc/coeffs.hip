@@ -3,18 +3,24 @@
 // Restates _coefficients_gto / _coefficients_polynomial / _add_self_terms
 // (soap_generator.py:1294-1435, 821-850) and the Cacc accumulation of the
 // closed-form derivative path (soap_generator.py:2414-2423) as one pass per
-// center:
+// center: one 256-thread workgroup per center walks the center's CSR edge
+// segment in tiles of 64 edges.
 //
-//   one 256-thread workgroup per center walks the center's CSR edge segment
-//   in tiles of 64 edges.  Phase A evaluates the tile: thread per edge for the
-//   real solid harmonics r^l Y_lm, thread per (edge, radial item) for the
-//   radial factors, both stored in LDS as [value][edge] rows (padded to 65 so
-//   that threads reading different rows hit different banks).  Phase B:
-//   wavefront w takes edges w, w+4, ... of the tile; each lane owns register
-//   accumulators for (2 radial n) x (4 consecutive lm of one l) x (up to 4
-//   species, chosen by a wavefront-uniform branch on the edge's species).
-//   The four wavefronts' partial sums are combined in LDS in a fixed order, so
-//   results are bitwise reproducible.
+// Phase A (one pass, no barrier inside): in every wavefront lane e holds edge e
+// of the tile and recomputes its geometry (the repeated gathers hit the cache).
+// Wavefront q evaluates the harmonics of orders m = q (mod 4) and the radial
+// functions of degrees l = q (mod 4) (polynomial basis: radial index k = q
+// mod 4), so the four wavefronts split each edge's work without divergence.
+// Results go to LDS rows per edge: harmonics in groups of 4 aligned to 16 bytes
+// (one group = 4 consecutive m of one l), radial values [l][n] with n padded to
+// even.  Every wavefront also holds, from ballots, a 64-bit mask per species
+// of the tile's edges that count.
+// Phase B: lane (n pair, lm group) of wavefront w accumulates (2 n) x (4 lm) x
+// (species) for the tile's edges e = w (mod 4): a scalar loop over the set
+// bits of each species mask (no per-edge branch, no wasted products), one
+// 8-byte radial and one 16-byte harmonic LDS read per edge.
+// The four wavefronts' partial sums are combined in LDS in a fixed order, so
+// results are bitwise reproducible.
 //
 // Arithmetic by precision:
 //   F64    harmonics, radial, accumulation in double; GTO accumulates the
@@ -29,7 +35,10 @@
 // already orthonormal (no beta).
 // Radial table (basis.rad_table, the learned radial model): B_nl(r) by cubic
 // Hermite interpolation of the tabulated values and slopes, already orthonormal;
-// float accumulation except for F64.
+// float accumulation except for F64.  The table is read with the lanes of a
+// wavefront on the (l, n) channels of one edge (coalesced node rows), not one
+// edge per lane, which would scatter every lane over its own rows.
+// GTO constants (gamma, pref, beta) are staged in LDS once per workgroup.
 //
 // create_path: edges with |r| <= 1e-12 are skipped and the l=0 self term is
 // added for index centers, as the reference does; otherwise every edge counts.
@@ -40,13 +49,19 @@ namespace cusoap {
 namespace {
 
 constexpr int kRbfTable = 2;     // kernel variant: tabulated radial model
-constexpr int kTE = 64;          // edges per tile
-constexpr int kTEP = kTE + 1;    // padded LDS row length
+constexpr int kTE = 64;          // edges per tile (one per lane)
 constexpr int kThreads = 256;
 
 __host__ __device__ constexpr int n_groups(int lmax) {
     int g = 0;
     for (int l = 0; l <= lmax; ++l) g += (2 * l + 1 + 3) / 4;
+    return g;
+}
+
+// first lm group of degree l
+__host__ __device__ constexpr int group_start(int l) {
+    int g = 0;
+    for (int k = 0; k < l; ++k) g += (2 * k + 1 + 3) / 4;
     return g;
 }
 
@@ -78,8 +93,31 @@ __device__ __forceinline__ float t_exp<float>(float v) { return expf(v); }
 template <>
 __device__ __forceinline__ double t_exp<double>(double v) { return exp(v); }
 
+// LDS constants of the GTO basis, rounded to 16 bytes
+__host__ __device__ inline int gcon_doubles(bool gto, int L1, int n) { return gto ? ((L1 * n * (2 + n) + 1) & ~1) : 0; }
+
+// LDS row strides (elements): harmonics 4 NG + 4, radial (l_max+1) n_even + 2
+__host__ __device__ inline int y_stride(int lmax) { return 4 * n_groups(lmax) + 4; }
+__host__ __device__ inline int r_stride(int lmax, int n) { return (lmax + 1) * ((n + 1) & ~1) + 2; }
+
+template <typename T>
+struct V2;
+template <>
+struct V2<float> {
+    using type = float2;
+};
+template <>
+struct V2<double> {
+    using type = double2;
+};
+
+// 4 wavefronts per SIMD for the float accumulators (a few spilled registers cost
+// less than the lost occupancy); 3 for float64, which would spill more
+template <int PREC>
+constexpr int coeff_min_waves() { return PREC == CUSOAP_PREC_F64 ? 3 : 4; }
+
 template <int LMAX, int PREC, int RBF>
-__global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
+__global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
                                                          const int32_t* __restrict__ atom_sp,
                                                          const double* __restrict__ shifts,
                                                          const double* __restrict__ centers,
@@ -91,34 +129,51 @@ __global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const
     using TA = typename CT<PREC>::TA;
     using TQ = typename CT<PREC>::TQ;
     constexpr bool kBetaPerCenter = (RBF == CUSOAP_RBF_GTO) && (PREC != CUSOAP_PREC_MIXED);
+    constexpr bool kGtoConst = RBF == CUSOAP_RBF_GTO;
     constexpr int L1 = LMAX + 1, L2 = L1 * L1, NG = n_groups(LMAX);
-    const int n = b.n_max, S = b.n_species, NL = n * L1;
+    const int n = b.n_max, S = b.n_species;
+    const int nP = (n + 1) & ~1;
+    const int YS = y_stride(LMAX), RS = r_stride(LMAX, n);
     extern __shared__ __align__(16) unsigned char smem[];
-    double* geo = (double*)smem;                   // [kTE][8]: rx ry rz r2 r w
-    int* spe = (int*)(geo + kTE * 8);              // [kTE] species, -1 = skip
-    TA* Rs = (TA*)(spe + kTE);                     // [NL][kTEP]
-    TY* Ys = (TY*)(Rs + NL * kTEP + 2);            // [L2][kTEP]
+    double* gcon = (double*)smem;                     // gamma[L1 n], pref[L1 n], beta[L1 n n] (GTO)
+    TY* Ys = (TY*)(gcon + gcon_doubles(kGtoConst, L1, n));   // [kTE][YS]
+    TA* Rs = (TA*)(Ys + kTE * YS);                    // [kTE][RS]; later the combine / beta stage
 
     const int c = blockIdx.x, tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63;
+    const int q = __builtin_amdgcn_readfirstlane(wv);
+    if (kGtoConst) {
+        const int nc = L1 * n;
+        for (int i = tid; i < nc; i += kThreads) {
+            gcon[i] = b.gto_gamma[i];
+            gcon[nc + i] = b.gto_pref[i];
+        }
+        if (PREC == CUSOAP_PREC_MIXED)
+            for (int i = tid; i < nc * n; i += kThreads) gcon[2 * nc + i] = b.gto_beta[i];
+    }
     const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
     const int64_t e0 = offsets[c], e1 = offsets[c + 1];
-    const int wv = tid >> 6, lane = tid & 63;
     const int npair = (n + 1) / 2;
     const int nslot_all = npair * NG;                 // (n pair, lm group) slots per species
     const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
     const int cslot = (RBF == kRbfTable && self_sp[c] >= 0) ? self_sp[c] : S;   // table slot of the center
+    const double* g_gamma = gcon;
+    const double* g_pref = gcon + L1 * n;
+    const double* g_beta = gcon + 2 * L1 * n;
+    __syncthreads();
 
     for (int s0 = 0; s0 < S; s0 += 4) {
         const int s1 = (s0 + 4 < S) ? s0 + 4 : S;
         for (int q0 = 0; q0 < nslot_all; q0 += 64) {
             // ---- this lane's slot: n pair p (n0 = 2p, n1 = 2p+1) x lm group g of degree l
-            const int q = q0 + lane;
-            const bool valid = q < nslot_all;
-            int n0 = 0, l = 0, lm0 = 0, cnt = 0;
+            const int qs = q0 + lane;
+            const bool valid = qs < nslot_all;
+            int n0 = 0, l = 0, lm0 = 0, cnt = 0, grp = 0;
             bool has1 = false;
             if (valid) {
-                const int pp = q / NG;
-                int g = q - pp * NG;
+                const int pp = qs / NG;
+                grp = qs - pp * NG;
+                int g = grp;
                 while (g >= (2 * l + 1 + 3) / 4) {
                     g -= (2 * l + 1 + 3) / 4;
                     ++l;
@@ -129,7 +184,7 @@ __global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const
                 const int left = 2 * l + 1 - 4 * g;
                 cnt = left < 4 ? left : 4;
             }
-            const int r0i = (n0 * L1 + l) * kTEP, r1i = ((n0 + 1) * L1 + l) * kTEP, yi = lm0 * kTEP;
+            const int roff = l * nP + n0, yoff = 4 * grp;
             TA acc[4][2][4];
 #pragma unroll
             for (int a4 = 0; a4 < 4; ++a4)
@@ -140,124 +195,129 @@ __global__ __launch_bounds__(kThreads) void coeff_kernel(cusoap_basis_t b, const
 
             for (int64_t t0 = e0; t0 < e1; t0 += kTE) {
                 const int nt = (int)((e1 - t0) < kTE ? (e1 - t0) : kTE);
-                // ---- A1: geometry
-                if (tid < nt) {
-                    const int ext = nbr[t0 + tid];
+                // ---- A: lane = edge; wavefront q takes m = q, l = q (mod 4)
+                int sp = -1;
+                double rx = 0.0, ry = 0.0, rz = 0.0, r2 = 0.0, r = 0.0, w = 1.0;
+                if (lane < nt) {
+                    const int ext = nbr[t0 + lane];
                     const int img = ext / n_atoms, a = ext - img * n_atoms;
-                    const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
-                    const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
-                    const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
-                    const double r2 = (rx * rx + ry * ry) + rz * rz;
-                    const double r = sqrt(r2);
-                    double w = 1.0, dw = 0.0;
-                    if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
-                    double* g = geo + tid * 8;
-                    g[0] = rx; g[1] = ry; g[2] = rz; g[3] = r2; g[4] = r; g[5] = w;
-                    const int sp = atom_sp[a];
-                    spe[tid] = ((create_path && !(r > 1e-12)) || sp < s0 || sp >= s1) ? -1 : sp;
-                }
-                __syncthreads();
-                // ---- A2: harmonics (item per edge) and radial factors
-                int n_rad;
-                if (RBF == CUSOAP_RBF_POLY) n_rad = nt * n;
-                else if (RBF == kRbfTable) n_rad = nt * L1;
-                else if (kBetaPerCenter) n_rad = nt * NL;
-                else n_rad = nt * L1;
-                for (int item = tid; item < nt + n_rad; item += kThreads) {
-                    if (item < nt) {
-                        if (spe[item] < 0) continue;
-                        const double* g = geo + item * 8;
-                        TY* y = Ys + item;
-                        solid_harmonics<LMAX, false, TY>((TY)g[0], (TY)g[1], (TY)g[2],
-                                                         [&](int lm, TY v, TY, TY, TY) { y[lm * kTEP] = v; });
-                        continue;
+                    rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
+                    ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
+                    rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
+                    r2 = (rx * rx + ry * ry) + rz * rz;
+                    r = sqrt(r2);
+                    if (b.weighting != CUSOAP_W_NONE) {
+                        double dw = 0.0;
+                        weighting(b, r, w, dw);
                     }
-                    const int it = item - nt;
-                    if (RBF == kRbfTable) {
-                        const int ll = it / nt, e = it - ll * nt;
-                        if (spe[e] < 0) continue;
-                        const int G = b.rad_intervals;
-                        const double xr = geo[e * 8 + 4] * ((double)G / b.rad_rmax);
+                    const int spa = atom_sp[a];
+                    sp = ((create_path && !(r > 1e-12)) || spa < s0 || spa >= s1) ? -1 : spa;
+                }
+                uint64_t msk[4];
+#pragma unroll
+                for (int a4 = 0; a4 < 4; ++a4) msk[a4] = __ballot(sp == s0 + a4);
+                if (RBF == kRbfTable) {
+                    // radial table: lanes = (l, n) channels of one edge, so each edge reads its
+                    // two node rows with coalesced loads; wavefront q takes edges e = q (mod 4)
+                    uint64_t todo = __ballot(sp >= 0) & (0x1111111111111111ull << q);
+                    const int G = b.rad_intervals, NS = L1 * 2 * n;
+                    const double inv_h = (double)G / b.rad_rmax;
+                    const TA hh = (TA)(b.rad_rmax / G);
+                    while (todo) {
+                        const int e = __builtin_ctzll(todo);
+                        todo &= todo - 1;
+                        const double re = __shfl(r, e, 64), we = __shfl(w, e, 64);
+                        const int se = __builtin_amdgcn_readlane(sp, e);
+                        const double xr = re * inv_h;
                         int i = (int)xr;
                         i = i < G - 1 ? i : G - 1;
-                        const TA t = (TA)(xr - i), hh = (TA)(b.rad_rmax / G);
+                        const TA t = (TA)(xr - i);
                         const TA t2 = t * t, t3 = t2 * t;
                         const TA h00 = TA(2) * t3 - TA(3) * t2 + TA(1), h01 = TA(3) * t2 - TA(2) * t3;
                         const TA h10 = hh * (t3 - TA(2) * t2 + t), h11 = hh * (t3 - t2);
-                        const int NS = L1 * 2 * n;
-                        const float* p0 = b.rad_table + ((size_t)(cslot * S + spe[e]) * (G + 1) + i) * NS + ll * 2 * n;
-                        const float* p1 = p0 + NS;
-                        const TA w = (TA)geo[e * 8 + 5];
-                        for (int nn = 0; nn < n; ++nn) {
-                            const TA B = h00 * (TA)p0[nn] + h10 * (TA)p0[n + nn] + h01 * (TA)p1[nn] + h11 * (TA)p1[n + nn];
-                            Rs[(nn * L1 + ll) * kTEP + e] = w * B;
+                        const float* p0 = b.rad_table + ((size_t)(cslot * S + se) * (G + 1) + i) * NS;
+                        for (int ch = lane; ch < L1 * n; ch += 64) {
+                            const int ll = ch / n, nn = ch - ll * n;
+                            const float* a0 = p0 + ll * 2 * n + nn;
+                            const TA B = h00 * (TA)a0[0] + h10 * (TA)a0[n] + h01 * (TA)a0[NS] + h11 * (TA)a0[NS + n];
+                            Rs[e * RS + ll * nP + nn] = (TA)we * B;
                         }
-                    } else if (RBF == CUSOAP_RBF_GTO && kBetaPerCenter) {
-                        const int t = it / nt, e = it - t * nt;
-                        if (spe[e] < 0) continue;
-                        const int k = t / L1, ll = t - k * L1;
-                        const TQ pref = (TQ)b.gto_pref[ll * n + k], gam = (TQ)b.gto_gamma[ll * n + k];
-                        Rs[t * kTEP + e] = (TA)((TQ)geo[e * 8 + 5] * (pref * t_exp<TQ>(-gam * (TQ)geo[e * 8 + 3])));
-                    } else if (RBF == CUSOAP_RBF_GTO) {
-                        const int ll = it / nt, e = it - ll * nt;
-                        if (spe[e] < 0) continue;
-                        const double r2 = geo[e * 8 + 3], w = geo[e * 8 + 5];
-                        double qv[kMaxN];
+                    }
+                }
+                if (sp >= 0) {
+                    TY* yrow = Ys + lane * YS;
+                    solid_harmonics_stripe<LMAX, TY>((TY)rx, (TY)ry, (TY)rz, q, [&](int ll, int m, TY v) {
+                        yrow[4 * group_start(ll) + ll + m] = v;
+                    });
+                    TA* rrow = Rs + lane * RS;
+                    if (RBF == kRbfTable) {
+                        // radial values written above
+                    } else if (RBF == CUSOAP_RBF_POLY) {
+                        for (int k = q; k < n; k += 4) {
+                            double I[L1];
 #pragma unroll
-                        for (int k = 0; k < kMaxN; ++k)
-                            qv[k] = k < n ? b.gto_pref[ll * n + k] * exp(-b.gto_gamma[ll * n + k] * r2) : 0.0;
-                        for (int nn = 0; nn < n; ++nn) {
-                            const double* beta = b.gto_beta + ((size_t)ll * n + nn) * n;
-                            double B = 0.0;
+                            for (int ll = 0; ll < L1; ++ll) I[ll] = 0.0;
+                            if (!create_path && r < 1e-4) {
 #pragma unroll
-                            for (int k = 0; k < kMaxN; ++k)
-                                if (k < n) B += beta[k] * qv[k];
-                            Rs[(nn * L1 + ll) * kTEP + e] = (TA)(w * B);
+                                for (int ll = 0; ll < L1; ++ll) I[ll] = b.poly_B0[k * L1 + ll];
+                            } else {
+                                const int Q = b.quad_n;
+                                poly_levels(b, r, LMAX, [&](int qq, int ll, double P) { I[ll] += P * b.poly_gss[k * Q + qq]; });
+                                double rl = 1.0;
+#pragma unroll
+                                for (int ll = 0; ll < L1; ++ll) {
+                                    I[ll] /= rl;
+                                    rl *= r;
+                                }
+                            }
+#pragma unroll
+                            for (int ll = 0; ll < L1; ++ll) rrow[ll * nP + k] = (TA)(w * I[ll]);
                         }
                     } else {
-                        const int k = it / nt, e = it - k * nt;
-                        if (spe[e] < 0) continue;
-                        const double r = geo[e * 8 + 4], w = geo[e * 8 + 5];
-                        double I[L1];
 #pragma unroll
-                        for (int ll = 0; ll < L1; ++ll) I[ll] = 0.0;
-                        if (!create_path && r < 1e-4) {
+                        for (int ll = 0; ll < L1; ++ll) {
+                            if ((ll & 3) != q) continue;
+                            if (kBetaPerCenter) {
+                                for (int k = 0; k < n; ++k) {
+                                    const TQ pref = (TQ)g_pref[ll * n + k], gam = (TQ)g_gamma[ll * n + k];
+                                    rrow[ll * nP + k] = (TA)((TQ)w * (pref * t_exp<TQ>(-gam * (TQ)r2)));
+                                }
+                            } else {
+                                double qv[kMaxN];
 #pragma unroll
-                            for (int ll = 0; ll < L1; ++ll) I[ll] = b.poly_B0[k * L1 + ll];
-                        } else {
-                            const int Q = b.quad_n;
-                            poly_levels(b, r, LMAX, [&](int qq, int ll, double P) { I[ll] += P * b.poly_gss[k * Q + qq]; });
-                            double rl = 1.0;
+                                for (int k = 0; k < kMaxN; ++k)
+                                    qv[k] = k < n ? g_pref[ll * n + k] * exp(-g_gamma[ll * n + k] * r2) : 0.0;
+                                for (int nn = 0; nn < n; ++nn) {
+                                    const double* beta = g_beta + ((size_t)ll * n + nn) * n;
+                                    double B = 0.0;
 #pragma unroll
-                            for (int ll = 0; ll < L1; ++ll) {
-                                I[ll] /= rl;
-                                rl *= r;
+                                    for (int k = 0; k < kMaxN; ++k)
+                                        if (k < n) B += beta[k] * qv[k];
+                                    rrow[ll * nP + nn] = (TA)(w * B);
+                                }
                             }
                         }
-#pragma unroll
-                        for (int ll = 0; ll < L1; ++ll) Rs[(k * L1 + ll) * kTEP + e] = (TA)(w * I[ll]);
                     }
                 }
                 __syncthreads();
-                // ---- B: wavefront wv sweeps edges wv, wv+4, ...; species is wavefront-uniform
+                // ---- B: wavefront wv sums its edges e = wv (mod 4), species by species
                 if (valid) {
-                    for (int e = wv; e < nt; e += 4) {
-                        const int se = __builtin_amdgcn_readfirstlane(spe[e]);
-                        if (se < 0) continue;
-                        const TA rv0 = Rs[r0i + e];
-                        const TA rv1 = has1 ? Rs[r1i + e] : TA(0);
-                        TA yv[4];
+                    const uint64_t mine = 0x1111111111111111ull << wv;
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) yv[j] = j < cnt ? (TA)Ys[yi + j * kTEP + e] : TA(0);
-                        const int sl = se - s0;
+                    for (int a4 = 0; a4 < 4; ++a4) {
+                        if (s0 + a4 >= s1) continue;
+                        uint64_t m = msk[a4] & mine;
+                        while (m) {
+                            const int e = __builtin_ctzll(m);
+                            m &= m - 1;
+                            const typename V2<TA>::type rv = *(const typename V2<TA>::type*)(Rs + e * RS + roff);
+                            const typename V2<TY>::type y01 = *(const typename V2<TY>::type*)(Ys + e * YS + yoff);
+                            const typename V2<TY>::type y23 = *(const typename V2<TY>::type*)(Ys + e * YS + yoff + 2);
+                            const TA yv[4] = {(TA)y01.x, (TA)y01.y, (TA)y23.x, (TA)y23.y};
 #pragma unroll
-                        for (int a4 = 0; a4 < 4; ++a4) {
-                            if (sl == a4) {
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) {
-                                    acc[a4][0][j] += rv0 * yv[j];
-                                    acc[a4][1][j] += rv1 * yv[j];
-                                }
+                            for (int j = 0; j < 4; ++j) {
+                                acc[a4][0][j] += rv.x * yv[j];
+                                acc[a4][1][j] += rv.y * yv[j];
                             }
                         }
                     }
@@ -331,12 +391,12 @@ int launch_coeff(const cusoap_basis_t& b, const double* pos, int n_atoms, const 
     using TY = typename CT<PREC>::TY;
     using TA = typename CT<PREC>::TA;
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
-    const int n = b.n_max, NL = n * L1;
-    const size_t tile = sizeof(double) * kTE * 8 + sizeof(int) * kTE + sizeof(TA) * ((size_t)NL * kTEP + 2) +
-                        sizeof(TY) * (size_t)L2 * kTEP;
-    const size_t cs = sizeof(TA) * (size_t)std::max(b.n_species, 4) * n * L2 + sizeof(double) * kTE * 8 +
-                      sizeof(int) * kTE;
-    const size_t smem = tile > cs ? tile : cs;
+    const int n = b.n_max;
+    const size_t ys_bytes = sizeof(TY) * (size_t)kTE * y_stride(LMAX);
+    const size_t rs_bytes = sizeof(TA) * (size_t)kTE * r_stride(LMAX, n);
+    const size_t part = sizeof(TA) * (size_t)std::max(b.n_species, 4) * n * L2;   // combine / beta stage (aliases Rs)
+    const size_t gcon = sizeof(double) * (size_t)gcon_doubles(RBF == CUSOAP_RBF_GTO, L1, n);
+    const size_t smem = gcon + ys_bytes + (part > rs_bytes ? part : rs_bytes);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: LDS tile too large");
     auto kern = coeff_kernel<LMAX, PREC, RBF>;
     if (smem > 64 * 1024)

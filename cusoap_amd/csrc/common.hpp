@@ -119,6 +119,48 @@ __device__ __forceinline__ void solid_harmonics(T x, T y, T z, Put&& put) {
 }
 
 
+// Values only, for the orders m = phase, phase + 4, phase + 8, ... (phase is
+// uniform across the wavefront, so four wavefronts split one edge's harmonics
+// without divergence).  put(l, m, v) with m signed (-l..l).
+template <int LMAX, typename T, typename Put>
+__device__ __forceinline__ void solid_harmonics_stripe(T x, T y, T z, int phase, Put&& put) {
+    const T r2 = (x * x + y * y) + z * z;
+    T cm = T(1), sm = T(0);          // (x+iy)^m
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) {
+        if (m > 0) {
+            const T nc = x * cm - y * sm;
+            const T ns = x * sm + y * cm;
+            cm = nc;
+            sm = ns;
+        }
+        if ((m & 3) != phase) continue;
+        T q2 = T(0), q1 = T(kDoubleFact[m]);
+#pragma unroll
+        for (int l = m; l <= LMAX; ++l) {
+            T q;
+            if (l == m) {
+                q = q1;
+            } else if (l == m + 1) {
+                q = T(2 * m + 1) * z * q1;
+            } else {
+                q = (T(2 * l - 1) * z * q1 - T(l + m - 1) * r2 * q2) * T(1.0 / (l - m));
+            }
+            if (l > m) {
+                q2 = q1;
+                q1 = q;
+            }
+            const T K = T(kSHNorm[l][m]);
+            if (m == 0) {
+                put(l, 0, K * q);
+            } else {
+                put(l, m, K * q * cm);
+                put(l, -m, K * q * sm);
+            }
+        }
+    }
+}
+
 // l-major variant of the same harmonics: on_l(l) once at the start of degree l,
 // then put(l, lm, v, gx, gy, gz) for m = -l..l (lm ascending).  Keeps the degree
 // l-1 and l-2 columns of the recurrence for every m in registers.
