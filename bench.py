@@ -48,15 +48,16 @@ def parse_args():
     ap.add_argument("--no-float64-reference", action="store_true",
                     help="(h2o_1000) skip timing the plain float64 form next to the chosen one")
     ap.add_argument("--config", default="h2o_1000", choices=["h2o_1000", "water_box", "hcno_box", "h2o_1m"])
-    ap.add_argument("--radial", default="exact", choices=["exact", "learned"],
-                    help="radial functions of the large configs: exact, or the learned model (trained first)")
+    ap.add_argument("--radial", default="exact", choices=["exact", "table", "learned"],
+                    help="radial functions of the large configs: exact per edge, the tabulated exact functions, "
+                         "or the learned model (trained first)")
     ap.add_argument("--radial-checkpoint", default="", metavar="NPZ",
                     help="learned radial networks: load this checkpoint if it exists, else train and save it")
     ap.add_argument("--spectrum", default="exact", choices=["exact", "fp16", "fp16x3", "bf16", "bf16x3"],
                     help="power-spectrum contraction of the large configs (matrix-core modes)")
     ap.add_argument("--compare", default="",
                     help="comma list of forms to compare against float64 (large configs): a precision, "
-                         "optionally with ':learned' and/or a spectrum mode, e.g. mixed:learned:bf16x3")
+                         "optionally with ':table' or ':learned' and a spectrum mode, e.g. mixed:learned:bf16x3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-centers", type=int, default=200)
     ap.add_argument("--check", action="store_true", help="compare the descriptor with the FP64 oracle")
@@ -267,7 +268,8 @@ def bench_adjoint(args, dev):
                               "radial_heldout_value_err": float(sur.errors["value"].max()),
                               "radial_heldout_slope_err": float(sur.errors["slope"].max())})
             else:
-                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, spectrum=spectrum, **params)
+                soaps[key] = SOAP(device=dev, dtype="float32", precision=prec, spectrum=spectrum, radial=radial,
+                                  **params)
         return soaps[key]
 
     soap = soap_for(args.precision, args.radial, args.spectrum)
@@ -299,7 +301,7 @@ def bench_adjoint(args, dev):
     out = {"metric": METRIC, "value": round(n_env * args.steps / dt, 1), "unit": "atom-envs/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": ("f32" if args.radial == "learned" and args.precision != "float64" else
+           "dtype": ("f32" if args.radial != "exact" and args.precision != "float64" else
                      {"float64": "f64", "mixed": "f64+f32", "float32": "f32"}[args.precision]),
            "data": "synthetic",
            "config": {"workload": desc_txt, "n_atoms": n_env, "n_features": n_feat, "precision": args.precision,
@@ -315,8 +317,8 @@ def bench_adjoint(args, dev):
         errs = {}
         for form in [p for p in args.compare.split(",") if p]:
             parts = form.split(":")
-            rad = "learned" if "learned" in parts[1:] else "exact"
-            spec = next((q for q in parts[1:] if q != "learned"), "exact")
+            rad = next((q for q in parts[1:] if q in ("learned", "table")), "exact")
+            spec = next((q for q in parts[1:] if q not in ("learned", "table")), "exact")
             dd, FF = step(soap_for(parts[0], rad, spec))
             dd = dd.double()
             sc = ref_d.abs().amax(0).clamp_min(1e-300)

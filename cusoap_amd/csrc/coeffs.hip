@@ -219,28 +219,54 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 if (RBF == kRbfTable) {
                     // radial table: lanes = (l, n) channels of one edge, so each edge reads its
                     // two node rows with coalesced loads; wavefront q takes edges e = q (mod 4)
+                    // four edges per round so that their sixteen row loads are in flight together
                     uint64_t todo = __ballot(sp >= 0) & (0x1111111111111111ull << q);
                     const int G = b.rad_intervals, NS = L1 * 2 * n;
                     const double inv_h = (double)G / b.rad_rmax;
                     const TA hh = (TA)(b.rad_rmax / G);
                     while (todo) {
-                        const int e = __builtin_ctzll(todo);
-                        todo &= todo - 1;
-                        const double re = __shfl(r, e, 64), we = __shfl(w, e, 64);
-                        const int se = __builtin_amdgcn_readlane(sp, e);
-                        const double xr = re * inv_h;
-                        int i = (int)xr;
-                        i = i < G - 1 ? i : G - 1;
-                        const TA t = (TA)(xr - i);
-                        const TA t2 = t * t, t3 = t2 * t;
-                        const TA h00 = TA(2) * t3 - TA(3) * t2 + TA(1), h01 = TA(3) * t2 - TA(2) * t3;
-                        const TA h10 = hh * (t3 - TA(2) * t2 + t), h11 = hh * (t3 - t2);
-                        const float* p0 = b.rad_table + ((size_t)(cslot * S + se) * (G + 1) + i) * NS;
-                        for (int ch = lane; ch < L1 * n; ch += 64) {
-                            const int ll = ch / n, nn = ch - ll * n;
-                            const float* a0 = p0 + ll * 2 * n + nn;
-                            const TA B = h00 * (TA)a0[0] + h10 * (TA)a0[n] + h01 * (TA)a0[NS] + h11 * (TA)a0[NS + n];
-                            Rs[e * RS + ll * nP + nn] = (TA)we * B;
+                        int ev[4];
+                        int ne = 0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            ev[j] = todo ? __builtin_ctzll(todo) : -1;
+                            if (todo) {
+                                todo &= todo - 1;
+                                ++ne;
+                            }
+                        }
+                        for (int ch0 = 0; ch0 < L1 * n; ch0 += 64) {
+                            const int ch = ch0 + lane;
+                            const bool on = ch < L1 * n;
+                            const int ll = on ? ch / n : 0, nn = on ? ch - ll * n : 0;
+                            float v[4][4];
+                            TA hw[4][4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int e = ev[j] < 0 ? ev[0] : ev[j];
+                                const double re = __shfl(r, e, 64), we = __shfl(w, e, 64);
+                                const int se = __builtin_amdgcn_readlane(sp, e);
+                                const double xr = re * inv_h;
+                                int i = (int)xr;
+                                i = i < G - 1 ? i : G - 1;
+                                const TA t = (TA)(xr - i);
+                                const TA t2 = t * t, t3 = t2 * t;
+                                hw[j][0] = (TA)we * (TA(2) * t3 - TA(3) * t2 + TA(1));
+                                hw[j][1] = (TA)we * hh * (t3 - TA(2) * t2 + t);
+                                hw[j][2] = (TA)we * (TA(3) * t2 - TA(2) * t3);
+                                hw[j][3] = (TA)we * hh * (t3 - t2);
+                                const float* a0 = b.rad_table + ((size_t)(cslot * S + se) * (G + 1) + i) * NS + ll * 2 * n + nn;
+                                v[j][0] = a0[0];
+                                v[j][1] = a0[n];
+                                v[j][2] = a0[NS];
+                                v[j][3] = a0[NS + n];
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                if (j >= ne || !on) continue;
+                                Rs[ev[j] * RS + ll * nP + nn] = hw[j][0] * (TA)v[j][0] + hw[j][1] * (TA)v[j][1] +
+                                                                hw[j][2] * (TA)v[j][2] + hw[j][3] * (TA)v[j][3];
+                            }
                         }
                     }
                 }

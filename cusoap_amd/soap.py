@@ -179,12 +179,16 @@ class SOAP:
     (float32 harmonics and accumulation, float64 radial functions) or
     "float32".  Output tensors are ``dtype`` as in the reference.
 
-    Extra keyword ``radial``: "exact" (default) or "learned".  "learned" trains
-    (or loads, ``surrogate=<checkpoint path>``) per-species-pair networks for the
-    GTO radial functions on the device.  The coefficient and adjoint kernels then
-    read a table of the gated model (cusoap_amd.surrogate).  ``surrogate`` may
-    also be a RadialSurrogate or a dict of its constructor and train() options.
-    The dense Jacobian always uses the exact radial functions.
+    Extra keyword ``radial``: "exact" (default), "table" or "learned".
+    "table" tabulates the exact float64 GTO radial functions and slopes once
+    (``table_intervals`` cubic-Hermite intervals on [0, cutoff]).  The coefficient
+    and adjoint kernels then interpolate per edge, with no exponentials and no
+    Loewdin contraction per edge.  "learned" trains (or loads,
+    ``surrogate=<checkpoint path>``) per-species-pair networks for the radial
+    functions on the device and tabulates the gated model the same way
+    (cusoap_amd.surrogate).  ``surrogate`` may also be a RadialSurrogate or a dict
+    of its constructor and train() options.  The dense Jacobian always uses the
+    exact radial functions.
 
     Extra keyword ``spectrum``: arithmetic of the power-spectrum contraction.
     "exact" (default) uses the vector units in the coefficient precision.
@@ -263,8 +267,8 @@ class SOAP:
         self._basis_struct = None
         self._basis_learned = None
         self._last_nl_backend = "hip-cell-list"
-        if radial not in ("exact", "learned"):
-            raise ValueError("radial must be 'exact' or 'learned'")
+        if radial not in ("exact", "table", "learned"):
+            raise ValueError("radial must be 'exact', 'table' or 'learned'")
         if spectrum not in _SPECTRA:
             raise ValueError(f"spectrum must be one of {sorted(_SPECTRA)}")
         self.spectrum = spectrum
@@ -273,6 +277,13 @@ class SOAP:
         self._rad_table = None
         if radial == "learned":
             self._init_learned_radial(surrogate, int(table_intervals))
+        elif radial == "table":
+            from .surrogate import radial_table
+            self._require_gpu()
+            if self._rbf != "gto":
+                raise NotImplementedError("radial='table' covers the GTO basis")
+            self._rad_intervals = int(table_intervals)
+            self._rad_table = radial_table(self, self._rad_intervals)
 
     def _init_learned_radial(self, surrogate, n_intervals: int):
         """Train or load the radial networks, gate them, tabulate the radial model."""

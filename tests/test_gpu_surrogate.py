@@ -20,7 +20,7 @@ import pytest
 import torch
 
 from cusoap_amd import SOAP, _lib, systems
-from cusoap_amd.surrogate import HIDDEN, RadialSurrogate, exact_radial, radial_table
+from cusoap_amd.surrogate import HIDDEN, exact_radial
 from oracle import mlp_oracle as mo
 from oracle import soap_oracle as so
 
@@ -192,15 +192,15 @@ def test_exact_table_interpolation_matches_oracle():
     s = systems.water_box(5, seed=4)
     p = dict(WB, periodic=True)
     o = so.OracleSOAP(**p)
-    g = SOAP(device=DEV, dtype="float64", precision="float64", _betas=o.betas, **p)
-    g._rad_table = radial_table(g, 1024)
-    g._rad_intervals = 1024
     w = np.random.default_rng(5).normal(size=o.get_number_of_features())
-    d, F = g.descriptor_and_vjp(s, torch.tensor(w))
     Fr = o.vjp(s, w[None, :].repeat(len(s["positions"]), 0))
-    ed, ef = colerr(d, o.create(s)), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
-    say(f"exact table: desc err {ed:.2e}, force err {ef:.2e}")
-    assert ed < 1e-6 and ef < 1e-5
+    ref = o.create(s)
+    for prec, tol in (("float64", (1e-6, 1e-5)), ("mixed", (1e-5, 1e-5))):
+        g = SOAP(device=DEV, dtype="float64", precision=prec, radial="table", _betas=o.betas, **p)
+        d, F = g.descriptor_and_vjp(s, torch.tensor(w))
+        ed, ef = colerr(d, ref), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
+        say(f"exact table ({prec}): desc err {ed:.2e}, force err {ef:.2e}")
+        assert ed < tol[0] and ef < tol[1]
 
 
 def test_learned_descriptor_and_forces_within_bound(learned):
