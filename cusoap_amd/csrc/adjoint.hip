@@ -43,7 +43,10 @@ __device__ __forceinline__ int cc_slot_of(int lm) {
     return lm;
 }
 
-// M[c][s][lm][k] (TM): dL/dc, with beta folded in when fold_beta
+// M[c][s][lm][k] (TM): dL/dc, with beta folded in when fold_beta.
+// Power spectrum: the workgroup first forms W_l[a][b] = (2 if a == b else 1)
+// g[feature(l, a, b)] in LDS from the symmetric feature table (coalesced reads),
+// then G[a][lm] = pref_l sum_b W_l[a][b] c[b][lm] from LDS only.
 template <typename TC, typename TG, typename TM>
 __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, int mode, const TC* __restrict__ coeffs,
                                                             const int32_t* __restrict__ pair_feature,
@@ -55,22 +58,35 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
     double* cs = (double*)smem;       // [per]   c[s][n][lm]
     double* G = cs + per;              // [per]   G[s][n][lm]
     double* g = G + per;               // [n_feat]
-    const int c = blockIdx.x;
+    double* W = g + n_feat;            // [L1][SN][SN] (power spectrum)
+    double* bs = W + (mode == CUSOAP_MODE_PS ? L1 * SN * SN : 0);   // [L1][n][n] beta (fold)
+    const int c = blockIdx.x, tid = threadIdx.x;
     const TC* src = coeffs + (size_t)c * per;
     const TG* gsrc = gd + (gd_broadcast ? 0 : (size_t)c * n_feat);
-    for (int i = threadIdx.x; i < per; i += blockDim.x) cs[i] = (double)src[i];
-    for (int i = threadIdx.x; i < n_feat; i += blockDim.x) g[i] = (double)gsrc[i];
+    for (int i = tid; i < per; i += blockDim.x) cs[i] = (double)src[i];
+    for (int i = tid; i < n_feat; i += blockDim.x) g[i] = (double)gsrc[i];
+    if (fold_beta)
+        for (int i = tid; i < L1 * n * n; i += blockDim.x) bs[i] = b.gto_beta[i];
     __syncthreads();
-    for (int t = threadIdx.x; t < per; t += blockDim.x) {
+    if (mode == CUSOAP_MODE_PS) {
+        __shared__ double prefl[kMaxL + 1];
+        if (tid < L1) prefl[tid] = sqrt(8.0 * M_PI * M_PI / (2 * tid + 1));
+        __syncthreads();
+        const int nw = L1 * SN * SN;
+        for (int i = tid; i < nw; i += blockDim.x) {
+            const int f = pair_feature[i];
+            const int l = i / (SN * SN), ab = i - l * SN * SN;
+            const int a = ab / SN, bb = ab - a * SN;
+            W[i] = f >= 0 ? (bb == a ? 2.0 : 1.0) * prefl[l] * g[f] : 0.0;
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < per; t += blockDim.x) {
         const int a = t / L2, lm = t - a * L2, l = lm_l2(lm);
         double v = 0.0;
         if (mode == CUSOAP_MODE_PS) {
-            const int32_t* pf = pair_feature + ((size_t)l * SN + a) * SN;
-            for (int bb = 0; bb < SN; ++bb) {
-                const int f = pf[bb];
-                if (f >= 0) v += (bb == a ? 2.0 : 1.0) * g[f] * cs[bb * L2 + lm];
-            }
-            v *= sqrt(8.0 * M_PI * M_PI / (2 * l + 1));
+            const double* wr = W + ((size_t)l * SN + a) * SN;
+            for (int bb = 0; bb < SN; ++bb) v += wr[bb] * cs[bb * L2 + lm];
         } else {
             const int slot = cc_slot_of(lm);
             v = g[a * L2 + slot] * (slot >= 4 ? 0.17958712212516656 : 1.0);
@@ -79,15 +95,16 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
     }
     __syncthreads();
     TM* Mc = M + (size_t)c * per;
-    for (int t = threadIdx.x; t < per; t += blockDim.x) {
+    for (int t = tid; t < per; t += blockDim.x) {
         // output order [s][lm][k]
         const int s = t / (L2 * n), rem = t - s * L2 * n;
         const int lm = rem / n, k = rem - lm * n;
         double v;
         if (fold_beta) {
             const int l = lm_l2(lm);
+            const double* br = bs + (size_t)l * n * n + k;
             v = 0.0;
-            for (int nn = 0; nn < n; ++nn) v += b.gto_beta[((size_t)l * n + nn) * n + k] * G[(s * n + nn) * L2 + lm];
+            for (int nn = 0; nn < n; ++nn) v += br[nn * n] * G[(s * n + nn) * L2 + lm];
         } else {
             v = G[(s * n + k) * L2 + lm];
         }
@@ -442,8 +459,10 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
 template <typename TC, typename TG, typename TM>
 int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const int32_t* pair_feature, const void* gd,
                    int gd_broadcast, int n_feat, int fold_beta, int n_centers, void* M, hipStream_t st) {
-    const int per = b.n_species * b.n_max * (b.l_max + 1) * (b.l_max + 1);
-    const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat);
+    const int L1 = b.l_max + 1, SN = b.n_species * b.n_max;
+    const int per = SN * L1 * L1;
+    const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat + (mode == CUSOAP_MODE_PS ? (size_t)L1 * SN * SN : 0) +
+                                          (fold_beta ? (size_t)L1 * b.n_max * b.n_max : 0));
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: descriptor row too large for LDS");
     auto kern = coeff_adjoint_kernel<TC, TG, TM>;
     if (smem > 64 * 1024)
