@@ -58,8 +58,9 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
     double* cs = (double*)smem;       // [per]   c[s][n][lm]
     double* G = cs + per;              // [per]   G[s][n][lm]
     double* g = G + per;               // [n_feat]
-    double* W = g + n_feat;            // [L1][SN][SN] (power spectrum)
-    double* bs = W + (mode == CUSOAP_MODE_PS ? L1 * SN * SN : 0);   // [L1][n][n] beta (fold)
+    const int WS = SN + 1;             // padded W row: lanes on different rows hit different banks
+    double* W = g + n_feat;            // [L1][SN][WS] (power spectrum)
+    double* bs = W + (mode == CUSOAP_MODE_PS ? L1 * SN * WS : 0);   // [L1][n][n] beta (fold)
     const int c = blockIdx.x, tid = threadIdx.x;
     const TC* src = coeffs + (size_t)c * per;
     const TG* gsrc = gd + (gd_broadcast ? 0 : (size_t)c * n_feat);
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
             const int f = pair_feature[i];
             const int l = i / (SN * SN), ab = i - l * SN * SN;
             const int a = ab / SN, bb = ab - a * SN;
-            W[i] = f >= 0 ? (bb == a ? 2.0 : 1.0) * prefl[l] * g[f] : 0.0;
+            W[((size_t)l * SN + a) * WS + bb] = f >= 0 ? (bb == a ? 2.0 : 1.0) * prefl[l] * g[f] : 0.0;
         }
         __syncthreads();
     }
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
         const int a = t / L2, lm = t - a * L2, l = lm_l2(lm);
         double v = 0.0;
         if (mode == CUSOAP_MODE_PS) {
-            const double* wr = W + ((size_t)l * SN + a) * SN;
+            const double* wr = W + ((size_t)l * SN + a) * WS;
             for (int bb = 0; bb < SN; ++bb) v += wr[bb] * cs[bb * L2 + lm];
         } else {
             const int slot = cc_slot_of(lm);
@@ -461,7 +462,7 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
                    int gd_broadcast, int n_feat, int fold_beta, int n_centers, void* M, hipStream_t st) {
     const int L1 = b.l_max + 1, SN = b.n_species * b.n_max;
     const int per = SN * L1 * L1;
-    const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat + (mode == CUSOAP_MODE_PS ? (size_t)L1 * SN * SN : 0) +
+    const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat + (mode == CUSOAP_MODE_PS ? (size_t)L1 * SN * (SN + 1) : 0) +
                                           (fold_beta ? (size_t)L1 * b.n_max * b.n_max : 0));
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: descriptor row too large for LDS");
     auto kern = coeff_adjoint_kernel<TC, TG, TM>;

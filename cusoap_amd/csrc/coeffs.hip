@@ -136,7 +136,9 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const int YS = y_stride(LMAX), RS = r_stride(LMAX, n);
     extern __shared__ __align__(16) unsigned char smem[];
     double* gcon = (double*)smem;                     // gamma[L1 n], pref[L1 n], beta[L1 n n] (GTO)
-    TY* Ys = (TY*)(gcon + gcon_doubles(kGtoConst, L1, n));   // [kTE][YS]
+    int64_t* tab_row = (int64_t*)(gcon + gcon_doubles(kGtoConst, L1, n));   // [kTE] table row (table)
+    TA* tab_w = (TA*)(tab_row + (RBF == kRbfTable ? kTE : 0));              // [kTE][4] weights (table)
+    TY* Ys = (TY*)(tab_w + (RBF == kRbfTable ? 4 * kTE : 0));               // [kTE][YS]
     TA* Rs = (TA*)(Ys + kTE * YS);                    // [kTE][RS]; later the combine / beta stage
 
     const int c = blockIdx.x, tid = threadIdx.x;
@@ -218,12 +220,27 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 for (int a4 = 0; a4 < 4; ++a4) msk[a4] = __ballot(sp == s0 + a4);
                 if (RBF == kRbfTable) {
                     // radial table: lanes = (l, n) channels of one edge, so each edge reads its
-                    // two node rows with coalesced loads; wavefront q takes edges e = q (mod 4)
-                    // four edges per round so that their sixteen row loads are in flight together
+                    // two node rows with coalesced loads; wavefront q takes edges e = q (mod 4),
+                    // four edges per round so that their sixteen row loads are in flight together.
+                    // The edge's own lane first stores its weighted Hermite factors and row offset.
                     uint64_t todo = __ballot(sp >= 0) & (0x1111111111111111ull << q);
                     const int G = b.rad_intervals, NS = L1 * 2 * n;
-                    const double inv_h = (double)G / b.rad_rmax;
-                    const TA hh = (TA)(b.rad_rmax / G);
+                    if (sp >= 0 && (lane & 3) == q) {
+                        const double xr = r * ((double)G / b.rad_rmax);
+                        int i = (int)xr;
+                        i = i < G - 1 ? i : G - 1;
+                        const TA t = (TA)(xr - i), hh = (TA)(b.rad_rmax / G), tw = (TA)w;
+                        const TA t2 = t * t, t3 = t2 * t;
+                        TA* hw = tab_w + lane * 4;
+                        hw[0] = tw * (TA(2) * t3 - TA(3) * t2 + TA(1));
+                        hw[1] = tw * hh * (t3 - TA(2) * t2 + t);
+                        hw[2] = tw * (TA(3) * t2 - TA(2) * t3);
+                        hw[3] = tw * hh * (t3 - t2);
+                        tab_row[lane] = ((int64_t)(cslot * S + sp) * (G + 1) + i) * NS;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     while (todo) {
                         int ev[4];
                         int ne = 0;
@@ -239,23 +256,12 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                             const int ch = ch0 + lane;
                             const bool on = ch < L1 * n;
                             const int ll = on ? ch / n : 0, nn = on ? ch - ll * n : 0;
+                            const int coff = ll * 2 * n + nn;
                             float v[4][4];
-                            TA hw[4][4];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
                                 const int e = ev[j] < 0 ? ev[0] : ev[j];
-                                const double re = __shfl(r, e, 64), we = __shfl(w, e, 64);
-                                const int se = __builtin_amdgcn_readlane(sp, e);
-                                const double xr = re * inv_h;
-                                int i = (int)xr;
-                                i = i < G - 1 ? i : G - 1;
-                                const TA t = (TA)(xr - i);
-                                const TA t2 = t * t, t3 = t2 * t;
-                                hw[j][0] = (TA)we * (TA(2) * t3 - TA(3) * t2 + TA(1));
-                                hw[j][1] = (TA)we * hh * (t3 - TA(2) * t2 + t);
-                                hw[j][2] = (TA)we * (TA(3) * t2 - TA(2) * t3);
-                                hw[j][3] = (TA)we * hh * (t3 - t2);
-                                const float* a0 = b.rad_table + ((size_t)(cslot * S + se) * (G + 1) + i) * NS + ll * 2 * n + nn;
+                                const float* a0 = b.rad_table + tab_row[e] + coff;
                                 v[j][0] = a0[0];
                                 v[j][1] = a0[n];
                                 v[j][2] = a0[NS];
@@ -264,8 +270,9 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
                                 if (j >= ne || !on) continue;
-                                Rs[ev[j] * RS + ll * nP + nn] = hw[j][0] * (TA)v[j][0] + hw[j][1] * (TA)v[j][1] +
-                                                                hw[j][2] * (TA)v[j][2] + hw[j][3] * (TA)v[j][3];
+                                const TA* hw = tab_w + ev[j] * 4;
+                                Rs[ev[j] * RS + ll * nP + nn] = hw[0] * (TA)v[j][0] + hw[1] * (TA)v[j][1] +
+                                                                hw[2] * (TA)v[j][2] + hw[3] * (TA)v[j][3];
                             }
                         }
                     }
@@ -422,7 +429,8 @@ int launch_coeff(const cusoap_basis_t& b, const double* pos, int n_atoms, const 
     const size_t rs_bytes = sizeof(TA) * (size_t)kTE * r_stride(LMAX, n);
     const size_t part = sizeof(TA) * (size_t)std::max(b.n_species, 4) * n * L2;   // combine / beta stage (aliases Rs)
     const size_t gcon = sizeof(double) * (size_t)gcon_doubles(RBF == CUSOAP_RBF_GTO, L1, n);
-    const size_t smem = gcon + ys_bytes + (part > rs_bytes ? part : rs_bytes);
+    const size_t tab = RBF == kRbfTable ? (sizeof(int64_t) + 4 * sizeof(TA)) * (size_t)kTE : 0;
+    const size_t smem = gcon + tab + ys_bytes + (part > rs_bytes ? part : rs_bytes);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: LDS tile too large");
     auto kern = coeff_kernel<LMAX, PREC, RBF>;
     if (smem > 64 * 1024)
