@@ -126,10 +126,27 @@ __device__ __forceinline__ double t_exp2<double>(double v) { return exp(v); }
 //      kRadTable M = G, B and Bt = (1/r) dB/dr by cubic Hermite interpolation of
 //                basis.rad_table (the derivative of the interpolant, so forward and
 //                adjoint describe the same function).
-// The radial vectors of the current degree live in LDS columns [k][thread].
+// The radial vectors of the current degree live in LDS columns [k][thread], or,
+// with a compile-time n_max (NN > 0), in registers, so that each lm needs only
+// the 16-byte reads of its M row.
 constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
 
-template <int LMAX, typename TM, int RAD>
+// M row of one lm into registers (16-byte reads when the row allows)
+template <typename T, int N>
+__device__ __forceinline__ void load_row(const T* p, T (&v)[N]) {
+    if constexpr (sizeof(T) == 4 && N % 4 == 0) {
+#pragma unroll
+        for (int k = 0; k < N; k += 4) {
+            const float4 q = *(const float4*)(p + k);
+            v[k] = q.x; v[k + 1] = q.y; v[k + 2] = q.z; v[k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = p[k];
+    }
+}
+
+template <int LMAX, typename TM, int RAD, int NN>
 __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
                                                       const int32_t* __restrict__ atom_sp,
                                                       const double* __restrict__ shifts,
@@ -142,14 +159,16 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     constexpr int NT = 256;
     constexpr bool MIXED = RAD == kRadMixed;
-    const int n = b.n_max, S = b.n_species;
+    constexpr bool FIXN = NN > 0;
+    constexpr int NR = FIXN ? NN : 1;
+    const int n = FIXN ? NN : b.n_max, S = b.n_species;
     int cslot = S;
     if (RAD == kRadTable && center_species[blockIdx.x] >= 0) cslot = center_species[blockIdx.x];
     extern __shared__ __align__(16) unsigned char smem[];
-    double* qd = (double*)smem;                             // [n][NT]   (MIXED) primitives
-    TM* rad0 = (TM*)(qd + (MIXED ? n * NT : 0));            // [n][NT]
-    TM* rad1 = rad0 + n * NT;                               // [n][NT]
-    TM* Ms = rad1 + n * NT;                                 // [S][L2][n]
+    double* qd = (double*)smem;                             // [n][NT]   (MIXED, runtime n) primitives
+    TM* rad0 = (TM*)(qd + (MIXED && !FIXN ? n * NT : 0));   // [n][NT]   (runtime n)
+    TM* rad1 = rad0 + (FIXN ? 0 : n * NT);                  // [n][NT]   (runtime n)
+    TM* Ms = rad1 + (FIXN ? 0 : n * NT);                    // [S][L2][n]
     __shared__ double red[3][4];
     const int c = blockIdx.x, tid = threadIdx.x;
     const int per = S * n * L2;
@@ -202,7 +221,48 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
             }
             tb = b.rad_table + ((size_t)(cslot * S + atom_sp[a]) * (G + 1) + i) * (L1 * 2 * n);
         }
+        TM rr0[NR], rr1[NR];                                 // radial vectors of degree l (FIXN)
         auto on_l = [&](int l) {
+            if constexpr (FIXN) {
+                const TM tw = (TM)w, tdwr = (TM)dwr;
+                if (RAD == kRadTable) {
+                    const float* p0 = tb + l * 2 * NN;
+                    const float* p1 = p0 + L1 * 2 * NN;
+#pragma unroll
+                    for (int k = 0; k < NN; ++k) {
+                        const TM b0 = (TM)p0[k], d0 = (TM)p0[NN + k], b1 = (TM)p1[k], d1 = (TM)p1[NN + k];
+                        const TM B = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
+                        const TM Bh = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
+                        rr0[k] = tw * B;
+                        rr1[k] = tw * Bh + tdwr * B;
+                    }
+                } else if (MIXED) {
+                    double qv[NN];
+#pragma unroll
+                    for (int k = 0; k < NN; ++k) qv[k] = b.gto_pref[l * NN + k] * exp(-b.gto_gamma[l * NN + k] * r2);
+#pragma unroll
+                    for (int nn = 0; nn < NN; ++nn) {
+                        const double* beta = b.gto_beta + ((size_t)l * NN + nn) * NN;
+                        double B = 0.0, Bh = 0.0;
+#pragma unroll
+                        for (int k = 0; k < NN; ++k) {
+                            B += beta[k] * qv[k];
+                            Bh += beta[k] * (-2.0 * b.gto_gamma[l * NN + k]) * qv[k];
+                        }
+                        rr0[nn] = (TM)(w * B);
+                        rr1[nn] = (TM)(w * Bh + dwr * B);
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NN; ++k) {
+                        const TM gam = (TM)b.gto_gamma[l * NN + k];
+                        const TM q = (TM)b.gto_pref[l * NN + k] * t_exp2<TM>(-gam * (TM)r2);
+                        rr0[k] = tw * q;
+                        rr1[k] = (tw * (TM(-2) * gam) + tdwr) * q;
+                    }
+                }
+                return;
+            }
             if (RAD == kRadTable) {
                 const float* p0 = tb + l * 2 * n;
                 const float* p1 = p0 + L1 * 2 * n;
@@ -251,10 +311,20 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
         auto put = [&](int, int lm, TM y, TM yx, TM yy, TM yz) {
             const TM* mv = Mrow + lm * n;
             TM h0 = TM(0), h1 = TM(0);
-            for (int k = 0; k < n; ++k) {
-                const TM m = mv[k];
-                h0 += r0c[k * NT] * m;
-                h1 += r1c[k * NT] * m;
+            if constexpr (FIXN) {
+                TM mk[NR];
+                load_row<TM, NR>(mv, mk);
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    h0 += rr0[k] * mk[k];
+                    h1 += rr1[k] * mk[k];
+                }
+            } else {
+                for (int k = 0; k < n; ++k) {
+                    const TM m = mv[k];
+                    h0 += r0c[k * NT] * m;
+                    h1 += r1c[k * NT] * m;
+                }
             }
             const TM yh1 = y * h1;
             gx += trx * yh1 + yx * h0;
@@ -408,15 +478,16 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
     }
 }
 
-template <int LMAX, typename TM, int RAD>
+template <int LMAX, typename TM, int RAD, int NN>
 int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
                    const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
                    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
                    double* grad, hipStream_t st) {
-    const size_t smem = sizeof(TM) * ((size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1) + 2 * 256 * (size_t)b.n_max) +
-                        (RAD == kRadMixed ? sizeof(double) * 256 * (size_t)b.n_max : 0);
+    const size_t cols = NN > 0 ? 0 : 2 * 256 * (size_t)b.n_max;
+    const size_t smem = sizeof(TM) * ((size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1) + cols) +
+                        (RAD == kRadMixed && NN == 0 ? sizeof(double) * 256 * (size_t)b.n_max : 0);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: coefficient block too large for LDS");
-    auto kern = vjp_gto_kernel<LMAX, TM, RAD>;
+    auto kern = vjp_gto_kernel<LMAX, TM, RAD, NN>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_edges", st);
@@ -433,14 +504,17 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
                      const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
                      double* grad, hipStream_t st) {
 #define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, st
+    const bool n8 = b.n_max == 8;         // registers for the radial vectors (float table form)
     if (b.rad_table) {
-        if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable>(CUSOAP_VJP_ARGS);
-        return launch_vjp_gto<LMAX, float, kRadTable>(CUSOAP_VJP_ARGS);
+        if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable, 0>(CUSOAP_VJP_ARGS);
+        return n8 ? launch_vjp_gto<LMAX, float, kRadTable, 8>(CUSOAP_VJP_ARGS)
+                  : launch_vjp_gto<LMAX, float, kRadTable, 0>(CUSOAP_VJP_ARGS);
     }
     if (b.rbf == CUSOAP_RBF_GTO) {
-        if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadPrim>(CUSOAP_VJP_ARGS);
-        if (prec == CUSOAP_PREC_F32) return launch_vjp_gto<LMAX, float, kRadPrim>(CUSOAP_VJP_ARGS);
-        return launch_vjp_gto<LMAX, float, kRadMixed>(CUSOAP_VJP_ARGS);
+        if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadPrim, 0>(CUSOAP_VJP_ARGS);
+        // (the register variant would exceed 256 VGPRs on these paths)
+        if (prec == CUSOAP_PREC_F32) return launch_vjp_gto<LMAX, float, kRadPrim, 0>(CUSOAP_VJP_ARGS);
+        return launch_vjp_gto<LMAX, float, kRadMixed, 0>(CUSOAP_VJP_ARGS);
     }
 #undef CUSOAP_VJP_ARGS
     const int L2 = (LMAX + 1) * (LMAX + 1);
