@@ -12,7 +12,7 @@
 //            reference's `nonzero` over (center, ext)
 // The distance test restates the reference formula
 //   d^2 = ((-2 c.n) + |c|^2) + |n|^2 <= cutoff^2
-// in float64 with explicit round-to-nearest operations (no contraction), so
+// in float64 with every operation rounded separately (no fused multiply-add), so
 // the edge set is the one that formula defines.
 #include <algorithm>
 #include <climits>
@@ -24,14 +24,19 @@ namespace cusoap {
 
 namespace {
 
+// Each product and sum is rounded on its own: contraction into fused multiply-adds
+// is switched off here (the HIP default would fuse them, which changes the edge
+// set at the cutoff boundary).
 __device__ __forceinline__ double sqnorm_rn(double x, double y, double z) {
-    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+#pragma clang fp contract(off)
+    return (x * x + y * y) + z * z;
 }
 
 __device__ __forceinline__ double dist2_formula(double cx, double cy, double cz, double cn,
                                                 double nx, double ny, double nz, double nn) {
-    const double dot = __dadd_rn(__dadd_rn(__dmul_rn(cx, nx), __dmul_rn(cy, ny)), __dmul_rn(cz, nz));
-    return __dadd_rn(__dadd_rn(__dmul_rn(dot, -2.0), cn), nn);
+#pragma clang fp contract(off)
+    const double dot = (cx * nx + cy * ny) + cz * nz;
+    return (dot * -2.0 + cn) + nn;
 }
 
 struct Grid {
@@ -344,10 +349,21 @@ __global__ __launch_bounds__(256) void count_kernel(cusoap_nl_plan_t plan, const
         cnt += edge_test(cx, cy, cz, cn, binned[cand_index(cr, k)], cutoff2, drop_self) ? 1 : 0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-    if (lane == 0) {
-        counts[c] = cnt;
-        atomicMax(max_count, cnt);
-    }
+    if (lane == 0) counts[c] = cnt;
+}
+
+// largest per-center count: one atomic per workgroup (a per-center atomic on one
+// address serialises, ~1 ms per 10^5 centers)
+__global__ __launch_bounds__(256) void max_count_kernel(const int32_t* __restrict__ counts, int n,
+                                                        int32_t* __restrict__ max_count) {
+    __shared__ int red[4];
+    int m = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, counts[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(max_count, max(max(red[0], red[1]), max(red[2], red[3])));
 }
 
 template <int CAP>
@@ -477,6 +493,7 @@ int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_ato
         prof_begin("nl_count", st);
         count_kernel<<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
                                                           w.cell_start, w.binned, w.counts, w.max_count);
+        max_count_kernel<<<std::min(256, (n_centers + 255) / 256), 256, 0, st>>>(w.counts, n_centers, w.max_count);
         prof_end("nl_count", st);
         CUSOAP_LAUNCH_CHECK();
     }

@@ -302,3 +302,39 @@ def test_descriptor_and_vjp_periodic_weighted_f64():
     d, F = g.descriptor_and_vjp(s, torch.tensor(G))
     np.testing.assert_allclose(F.cpu().numpy(), o.vjp(s, G), atol=1e-10 * np.abs(o.vjp(s, G)).max())
     assert colerr(d, o.create(s)) < RTOL64
+
+
+def _fused_d2(c, n):
+    """d^2 of the reference formula with the dot product fused (fma(cz, nz, fma(cy, ny, cx nx)))."""
+    from fractions import Fraction as Fr
+    p = float(Fr(c[0]) * Fr(n[0]))
+    p = float(Fr(c[1]) * Fr(n[1]) + Fr(p))
+    dot = float(Fr(c[2]) * Fr(n[2]) + Fr(p))
+    return ((dot * -2.0) + float(so.sqnorm3(c[None])[0])) + float(so.sqnorm3(n[None])[0])
+
+
+def test_nl_distance_rounding_at_cutoff():
+    """A pair whose separately rounded d^2 and fused-dot d^2 fall on opposite sides of cutoff^2
+    must be an edge exactly when the oracle's (separately rounded) formula says so."""
+    o = so.OracleSOAP(**W300)
+    cut2 = o.cutoff * o.cutoff
+    rng = np.random.default_rng(42)
+    found = None
+    for _ in range(200000):
+        c = rng.uniform(20.0, 60.0, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        n = c + d * o.cutoff * (1.0 + rng.uniform(-1e-13, 1e-13))
+        ci, ni = so.radius_edges(c[None], n[None], o.cutoff)
+        sep_in = len(ci) == 1
+        if sep_in != (_fused_d2(c, n) <= cut2):
+            found = (c, n, sep_in)
+            break
+    assert found is not None, "no straddling pair found"
+    c, n, sep_in = found
+    g = SOAP(device="cuda:0", dtype="float64", _betas=o.betas, **W300)
+    s = {"positions": np.stack([c, n]), "atomic_numbers": np.array([8, 1])}
+    say("straddling pair: oracle edge =", sep_in)
+    off, nbr = g.neighbor_list(s, drop_self=True)
+    counts = np.diff(off.cpu().numpy())
+    assert counts[0] == (1 if sep_in else 0)
