@@ -11,15 +11,20 @@
 //   g the edge gradient of the closed-form Jacobian (fixed centers):
 //   g[d][n][lm] = r_d Bt_nl Y_lm + Bw_nl dY_lm/dr_d.
 //
-// Kernel 1 (one workgroup per center) forms G and stores M[c][s][lm][k]: for
-// F64/F32 with the Loewdin matrix folded in, M = sum_n beta_l[n][k] G[s][n][lm],
-// so kernel 2 contracts with the primitive radial functions directly; for
-// MIXED, M = G and kernel 2 applies beta per edge in float64.
+// Kernel 1 forms G and stores M[c][s][lm][k]: for F64/F32 with the Loewdin
+// matrix folded in, M = sum_n beta_l[n][k] G[s][n][lm], so kernel 2 contracts
+// with the primitive radial functions directly; for MIXED and the radial table,
+// M = G.  For the GTO / table edge kernels M also carries the harmonic
+// normalisation K_lm.  With one descriptor gradient for all centers (the
+// linear-readout case) W_l is built once and a grid-stride kernel forms G from
+// it; otherwise one workgroup per center builds its own W_l.
 // Kernel 2 (GTO; 256 threads per center, thread per edge) walks the solid
-// harmonics l-major, builds the radial vectors of degree l once per degree in
-// per-thread LDS columns and contracts: H0 = sum_k rad0_k M[s][lm][k], H1 = sum_k rad1_k M[s][lm][k],
-// grad_d += r_d Y_lm H1 + dY_lm/dr_d H0; then scatters the 3-vector with
-// float64 atomics.  The polynomial basis uses a float64 tile kernel.
+// harmonics l-major, builds the radial vectors of degree l once per degree and
+// contracts: H0 = sum_k rad0_k M[s][lm][k], H1 = sum_k rad1_k M[s][lm][k]; the
+// gradient sum_lm [r H1 Y_lm + H0 grad Y_lm] is accumulated as r P + (X, Y, Z)
+// from the shared factors of each (l, |m|) pair (solid_harmonics_adjoint), and
+// the 3-vector is scattered with float64 atomics.  The polynomial basis uses a
+// float64 tile kernel.
 #include <cmath>
 
 #include "common.hpp"
@@ -33,6 +38,12 @@ __device__ __forceinline__ int lm_l2(int lm) {
     while (l * l > lm) --l;
     while ((l + 1) * (l + 1) <= lm) ++l;
     return l;
+}
+
+// K_lm of the real solid harmonic lm (folded into M for the GTO edge kernels)
+__device__ __forceinline__ double sh_norm_lm(int lm) {
+    const int l = lm_l2(lm), m = lm - l * l - l;
+    return kSHNorm[l][m < 0 ? -m : m];
 }
 
 // our lm -> DScribe cc slot holding it (inverse of the (z, x, y) permutation)
@@ -51,7 +62,7 @@ template <typename TC, typename TG, typename TM>
 __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, int mode, const TC* __restrict__ coeffs,
                                                             const int32_t* __restrict__ pair_feature,
                                                             const TG* __restrict__ gd, int gd_broadcast, int n_feat,
-                                                            int fold_beta, TM* __restrict__ M) {
+                                                            int fold_beta, int fold_k, TM* __restrict__ M) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int S = b.n_species, n = b.n_max, L1 = b.l_max + 1, L2 = L1 * L1;
     const int per = S * n * L2, SN = S * n;
@@ -109,7 +120,111 @@ __global__ __launch_bounds__(256) void coeff_adjoint_kernel(cusoap_basis_t b, in
         } else {
             v = G[(s * n + k) * L2 + lm];
         }
+        if (fold_k) v *= sh_norm_lm(lm);
         Mc[t] = (TM)v;
+    }
+}
+
+// LDS layout of W_l[a][b] in the shared-gradient kernel: odd row and degree
+// strides, so that the lanes of a wavefront (different l and a) spread over banks
+__host__ __device__ inline int psw_row(int SN) { return SN + 1 - (SN & 1); }
+__host__ __device__ inline int psw_deg(int SN) { return (SN * psw_row(SN)) | 1; }
+__host__ __device__ inline int psw_size(int L1, int SN) { return L1 * psw_deg(SN) + 3 * psw_row(SN); }
+
+// W_l[a][b] = (2 if a == b else 1) pref_l g[feature(l, a, b)] once for all centers
+// (descriptor gradient shared by every center, the linear-readout case), in the
+// padded layout above with three zero rows past the end (row-quad overrun).
+template <typename TG, typename TW>
+__global__ void ps_weights_kernel(int L1, int SN, const int32_t* __restrict__ pair_feature, const TG* __restrict__ gd,
+                                  TW* __restrict__ W) {
+    const int RS = psw_row(SN), LS = psw_deg(SN), nw = psw_size(L1, SN);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
+        const int l = i / LS, rem = i - l * LS, a = rem / RS, bb = rem - a * RS;
+        TW v = TW(0);
+        if (l < L1 && a < SN && bb < SN) {
+            const int f = pair_feature[((size_t)l * SN + a) * SN + bb];
+            const double prefl = sqrt(8.0 * M_PI * M_PI / (2 * l + 1));
+            if (f >= 0) v = (TW)((bb == a ? 2.0 : 1.0) * prefl * (double)gd[f]);
+        }
+        W[i] = v;
+    }
+}
+
+// Power-spectrum adjoint for a shared descriptor gradient.  A grid-stride loop
+// over centers; W (from ps_weights_kernel) is staged in LDS once per workgroup,
+// and the next center's coefficients are loaded into registers while the
+// current one is processed, so the loop is not one global-load latency per center.
+// Thread (lm, quad of rows a) forms G[a][lm] = sum_b W_l[a][b] c[b][lm] for four
+// a at once (one c read per four products), b ascending as in the per-center
+// kernel; then M is written in [s][lm][k] order (beta folded in when asked).
+// TW: arithmetic type (float64 when beta is folded or M is float64).
+constexpr int kPsPre = 8;        // prefetched values per thread (rows up to 2048 values)
+template <typename TC, typename TW, typename TM>
+__global__ __launch_bounds__(256) void ps_adjoint_shared_kernel(int S, int n, int L1, const TC* __restrict__ coeffs,
+                                                                const TW* __restrict__ Wg,
+                                                                const double* __restrict__ beta, int fold_beta,
+                                                                int fold_k, int n_centers, TM* __restrict__ M) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int SN = S * n, L2 = L1 * L1, per = SN * L2;
+    const int RS = psw_row(SN), LS = psw_deg(SN), nw = psw_size(L1, SN);
+    TW* W = (TW*)smem;                 // padded [L1][SN][SN] + 3 zero rows
+    TW* cs = W + nw;                   // [SN][L2]
+    TW* G = cs + per;                  // [SN][L2]
+    TW* bs = G + per;                  // [L1][n][n] (fold)
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nw; i += 256) W[i] = Wg[i];
+    if (fold_beta)
+        for (int i = tid; i < L1 * n * n; i += 256) bs[i] = (TW)beta[i];
+    const int nq = (SN + 3) / 4;
+    TC pre[kPsPre];
+    auto fetch = [&](int c) {
+        if (c >= n_centers) return;
+        const TC* src = coeffs + (size_t)c * per;
+#pragma unroll
+        for (int j = 0; j < kPsPre; ++j) {
+            const int i = tid + 256 * j;
+            if (i < per) pre[j] = src[i];
+        }
+    };
+    fetch(blockIdx.x);
+    for (int c = blockIdx.x; c < n_centers; c += gridDim.x) {
+#pragma unroll
+        for (int j = 0; j < kPsPre; ++j) {
+            const int i = tid + 256 * j;
+            if (i < per) cs[i] = (TW)pre[j];
+        }
+        __syncthreads();
+        fetch(c + gridDim.x);          // in flight while this center is processed
+        for (int t = tid; t < L2 * nq; t += 256) {
+            const int aq = t / L2, lm = t - aq * L2, l = lm_l2(lm), a0 = 4 * aq;
+            const TW* w0 = W + (size_t)l * LS + a0 * RS;
+            TW acc[4] = {TW(0), TW(0), TW(0), TW(0)};
+            for (int bb = 0; bb < SN; ++bb) {
+                const TW cv = cs[bb * L2 + lm];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] += w0[j * RS + bb] * cv;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (a0 + j < SN) G[(a0 + j) * L2 + lm] = acc[j];
+        }
+        __syncthreads();
+        TM* Mc = M + (size_t)c * per;
+        for (int t = tid; t < per; t += 256) {
+            const int s = t / (L2 * n), rem = t - s * L2 * n;
+            const int lm = rem / n, k = rem - lm * n;
+            TW v;
+            if (fold_beta) {
+                const TW* br = bs + (size_t)lm_l2(lm) * n * n + k;
+                v = TW(0);
+                for (int nn = 0; nn < n; ++nn) v += br[nn * n] * G[(s * n + nn) * L2 + lm];
+            } else {
+                v = G[(s * n + k) * L2 + lm];
+            }
+            if (fold_k) v *= (TW)sh_norm_lm(lm);
+            Mc[t] = (TM)v;
+        }
+        __syncthreads();
     }
 }
 
@@ -220,6 +335,16 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
                 c11 = (TM)((3.0 * t2 - 2.0 * t) * ir);
             }
             tb = b.rad_table + ((size_t)(cslot * S + atom_sp[a]) * (G + 1) + i) * (L1 * 2 * n);
+            // fold the weighting in: rad0 = w P(r), rad1 = w P'(r)/r + (w'/r) P(r)
+            const TM tw = (TM)w, tdwr = (TM)dwr;
+            c00 = tw * c00 + tdwr * v00;
+            c10 = tw * c10 + tdwr * v10;
+            c01 = tw * c01 + tdwr * v01;
+            c11 = tw * c11 + tdwr * v11;
+            v00 *= tw;
+            v10 *= tw;
+            v01 *= tw;
+            v11 *= tw;
         }
         TM rr0[NR], rr1[NR];                                 // radial vectors of degree l (FIXN)
         auto on_l = [&](int l) {
@@ -230,11 +355,9 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
                     const float* p1 = p0 + L1 * 2 * NN;
 #pragma unroll
                     for (int k = 0; k < NN; ++k) {
-                        const TM b0 = (TM)p0[k], d0 = (TM)p0[NN + k], b1 = (TM)p1[k], d1 = (TM)p1[NN + k];
-                        const TM B = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
-                        const TM Bh = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
-                        rr0[k] = tw * B;
-                        rr1[k] = tw * Bh + tdwr * B;
+                        const TM b0 = (TM)p0[2 * k], d0 = (TM)p0[2 * k + 1], b1 = (TM)p1[2 * k], d1 = (TM)p1[2 * k + 1];
+                        rr0[k] = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
+                        rr1[k] = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
                     }
                 } else if (MIXED) {
                     double qv[NN];
@@ -266,25 +389,19 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
             if (RAD == kRadTable) {
                 const float* p0 = tb + l * 2 * n;
                 const float* p1 = p0 + L1 * 2 * n;
-                const TM tw = (TM)w, tdwr = (TM)dwr;
                 auto emit = [&](int nn, float b0, float d0, float b1, float d1) {
-                    const TM B = v00 * (TM)b0 + v10 * (TM)d0 + v01 * (TM)b1 + v11 * (TM)d1;
-                    const TM Bh = c00 * (TM)b0 + c10 * (TM)d0 + c01 * (TM)b1 + c11 * (TM)d1;
-                    r0c[nn * NT] = tw * B;
-                    r1c[nn * NT] = tw * Bh + tdwr * B;
+                    r0c[nn * NT] = v00 * (TM)b0 + v10 * (TM)d0 + v01 * (TM)b1 + v11 * (TM)d1;
+                    r1c[nn * NT] = c00 * (TM)b0 + c10 * (TM)d0 + c01 * (TM)b1 + c11 * (TM)d1;
                 };
-                if ((n & 3) == 0) {
-                    // node rows start on 16-byte boundaries: four 16-byte loads per 4 functions
-                    for (int k4 = 0; k4 < n; k4 += 4) {
-                        const float4 b0 = *(const float4*)(p0 + k4), d0 = *(const float4*)(p0 + n + k4);
-                        const float4 b1 = *(const float4*)(p1 + k4), d1 = *(const float4*)(p1 + n + k4);
-                        emit(k4, b0.x, d0.x, b1.x, d1.x);
-                        emit(k4 + 1, b0.y, d0.y, b1.y, d1.y);
-                        emit(k4 + 2, b0.z, d0.z, b1.z, d1.z);
-                        emit(k4 + 3, b0.w, d0.w, b1.w, d1.w);
+                if ((n & 1) == 0) {
+                    // node rows start on 16-byte boundaries: two 16-byte loads per 2 functions
+                    for (int k2 = 0; k2 < n; k2 += 2) {
+                        const float4 a = *(const float4*)(p0 + 2 * k2), c = *(const float4*)(p1 + 2 * k2);
+                        emit(k2, a.x, a.y, c.x, c.y);
+                        emit(k2 + 1, a.z, a.w, c.z, c.w);
                     }
                 } else {
-                    for (int nn = 0; nn < n; ++nn) emit(nn, p0[nn], p0[n + nn], p1[nn], p1[n + nn]);
+                    for (int nn = 0; nn < n; ++nn) emit(nn, p0[2 * nn], p0[2 * nn + 1], p1[2 * nn], p1[2 * nn + 1]);
                 }
             } else if (MIXED) {
                 for (int k = 0; k < n; ++k) qc[k * NT] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);
@@ -308,9 +425,11 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
                 }
             }
         };
-        auto put = [&](int, int lm, TM y, TM yx, TM yy, TM yz) {
+        // the two radial contractions of lm with M (which carries K_lm)
+        auto hf = [&](int lm, TM& h0, TM& h1) {
             const TM* mv = Mrow + lm * n;
-            TM h0 = TM(0), h1 = TM(0);
+            h0 = TM(0);
+            h1 = TM(0);
             if constexpr (FIXN) {
                 TM mk[NR];
                 load_row<TM, NR>(mv, mk);
@@ -326,12 +445,12 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
                     h1 += r1c[k * NT] * m;
                 }
             }
-            const TM yh1 = y * h1;
-            gx += trx * yh1 + yx * h0;
-            gy += tryy * yh1 + yy * h0;
-            gz += trz * yh1 + yz * h0;
         };
-        solid_harmonics_lmajor<LMAX, true, TM>(trx, tryy, trz, on_l, put);
+        TM P = TM(0), X = TM(0), Y = TM(0), Z = TM(0);
+        solid_harmonics_adjoint<LMAX, TM>(trx, tryy, trz, on_l, hf, P, X, Y, Z);
+        gx = trx * P + X;
+        gy = tryy * P + Y;
+        gz = trz * P + Z;
         atomicAdd(&grad[3 * (size_t)a], (double)gx);
         atomicAdd(&grad[3 * (size_t)a + 1], (double)gy);
         atomicAdd(&grad[3 * (size_t)a + 2], (double)gz);
@@ -531,11 +650,50 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
     return CUSOAP_OK;
 }
 
+template <typename TC, typename TG, typename TW, typename TM>
+int launch_ps_shared(const cusoap_basis_t& b, const void* coeffs, const int32_t* pair_feature, const void* gd,
+                     int fold_beta, int fold_k, int n_centers, void* M, void* Wbuf, size_t smem, hipStream_t st) {
+    const int L1 = b.l_max + 1, SN = b.n_species * b.n_max;
+    const int nw = psw_size(L1, SN);
+    prof_begin("vjp_adjoint", st);
+    ps_weights_kernel<TG, TW><<<(nw + 255) / 256, 256, 0, st>>>(L1, SN, pair_feature, (const TG*)gd, (TW*)Wbuf);
+    CUSOAP_LAUNCH_CHECK();
+    auto kern = ps_adjoint_shared_kernel<TC, TW, TM>;
+    if (smem > 64 * 1024)
+        CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    const int grid = n_centers < 2048 ? n_centers : 2048;
+    kern<<<grid, 256, smem, st>>>(b.n_species, b.n_max, L1, (const TC*)coeffs, (const TW*)Wbuf, b.gto_beta, fold_beta,
+                                  fold_k, n_centers, (TM*)M);
+    prof_end("vjp_adjoint", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
+// bytes of M (sized for float64) at the head of the vjp workspace; W follows
+size_t vjp_m_bytes(const cusoap_basis_t* b, int n_centers) {
+    const int L2 = (b->l_max + 1) * (b->l_max + 1);
+    return align_up(sizeof(double) * (size_t)std::max(n_centers, 1) * b->n_species * b->n_max * L2, 256);
+}
+
+size_t vjp_w_bytes(const cusoap_basis_t* b) {
+    return align_up(sizeof(double) * (size_t)psw_size(b->l_max + 1, b->n_species * b->n_max), 256);
+}
+
 template <typename TC, typename TG, typename TM>
 int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const int32_t* pair_feature, const void* gd,
-                   int gd_broadcast, int n_feat, int fold_beta, int n_centers, void* M, hipStream_t st) {
+                   int gd_broadcast, int n_feat, int fold_beta, int fold_k, int n_centers, void* M, hipStream_t st) {
     const int L1 = b.l_max + 1, SN = b.n_species * b.n_max;
     const int per = SN * L1 * L1;
+    if (mode == CUSOAP_MODE_PS && gd_broadcast) {
+        // one weight matrix for all centers: the grid-stride kernel when it fits in LDS
+        const bool w64 = fold_beta || sizeof(TM) == 8;
+        const size_t tw = w64 ? 8 : 4;
+        const size_t smem = tw * ((size_t)psw_size(L1, SN) + 2 * (size_t)per + (fold_beta ? (size_t)L1 * b.n_max * b.n_max : 0));
+        void* Wbuf = (char*)M + vjp_m_bytes(&b, n_centers);
+        if (smem <= 160 * 1024 && per <= 256 * kPsPre)
+            return w64 ? launch_ps_shared<TC, TG, double, TM>(b, coeffs, pair_feature, gd, fold_beta, fold_k, n_centers, M, Wbuf, smem, st)
+                       : launch_ps_shared<TC, TG, float, TM>(b, coeffs, pair_feature, gd, fold_beta, fold_k, n_centers, M, Wbuf, smem, st);
+    }
     const size_t smem = sizeof(double) * (2 * (size_t)per + n_feat + (mode == CUSOAP_MODE_PS ? (size_t)L1 * SN * (SN + 1) : 0) +
                                           (fold_beta ? (size_t)L1 * b.n_max * b.n_max : 0));
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: descriptor row too large for LDS");
@@ -544,7 +702,7 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_adjoint", st);
     kern<<<n_centers, 256, smem, st>>>(b, mode, (const TC*)coeffs, pair_feature, (const TG*)gd, gd_broadcast, n_feat,
-                                       fold_beta, (TM*)M);
+                                       fold_beta, fold_k, (TM*)M);
     prof_end("vjp_adjoint", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -552,10 +710,7 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
 
 }  // namespace
 
-size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) {
-    const int L2 = (b->l_max + 1) * (b->l_max + 1);
-    return align_up(sizeof(double) * (size_t)std::max(n_centers, 1) * b->n_species * b->n_max * L2, 256);
-}
+size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) { return vjp_m_bytes(b, n_centers) + vjp_w_bytes(b); }
 
 int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
              const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
@@ -574,9 +729,10 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
         return fail(CUSOAP_ERR_ARG, "vjp: radial table needs center_species and a valid grid");
     if (b->rbf != CUSOAP_RBF_GTO && !b->rad_table) prec = CUSOAP_PREC_F64;   // the polynomial path is float64
     const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED && !b->rad_table) ? 1 : 0;
+    const int fold_k = (b->rbf == CUSOAP_RBF_GTO || b->rad_table) ? 1 : 0;   // the GTO / table edge kernels
     int rc;
 #define CUSOAP_ADJ(TC, TG, TM) \
-    launch_adjoint<TC, TG, TM>(*b, mode, coeffs, pair_feature, gd, gd_broadcast, n_feat, fold, n_centers, ws, st)
+    launch_adjoint<TC, TG, TM>(*b, mode, coeffs, pair_feature, gd, gd_broadcast, n_feat, fold, fold_k, n_centers, ws, st)
     const bool c64 = coeffs_dtype == CUSOAP_DT_F64, g64 = gd_dtype == CUSOAP_DT_F64;
     if (prec == CUSOAP_PREC_F64) {
         if (c64) rc = g64 ? CUSOAP_ADJ(double, double, double) : CUSOAP_ADJ(double, float, double);

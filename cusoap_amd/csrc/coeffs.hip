@@ -15,10 +15,13 @@
 // (one group = 4 consecutive m of one l), radial values [l][n] with n padded to
 // even.  Every wavefront also holds, from ballots, a 64-bit mask per species
 // of the tile's edges that count.
+// The tile's counted edges are compacted by species (ballot + mbcnt ranks), and
+// the LDS rows are indexed by that position, so every loop below runs over a
+// contiguous, wavefront-uniform range.
 // Phase B: lane (n pair, lm group) of wavefront w accumulates (2 n) x (4 lm) x
-// (species) for the tile's edges e = w (mod 4): a scalar loop over the set
-// bits of each species mask (no per-edge branch, no wasted products), one
-// 8-byte radial and one 16-byte harmonic LDS read per edge.
+// (species) for the positions p = w (mod 4) of each species range (no per-edge
+// branch, no wasted products), one 8-byte radial and one 16-byte harmonic LDS
+// read per edge.
 // The four wavefronts' partial sums are combined in LDS in a fixed order, so
 // results are bitwise reproducible.
 //
@@ -136,9 +139,9 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const int YS = y_stride(LMAX), RS = r_stride(LMAX, n);
     extern __shared__ __align__(16) unsigned char smem[];
     double* gcon = (double*)smem;                     // gamma[L1 n], pref[L1 n], beta[L1 n n] (GTO)
-    int64_t* tab_row = (int64_t*)(gcon + gcon_doubles(kGtoConst, L1, n));   // [kTE] table row (table)
-    TA* tab_w = (TA*)(tab_row + (RBF == kRbfTable ? kTE : 0));              // [kTE][4] weights (table)
-    TY* Ys = (TY*)(tab_w + (RBF == kRbfTable ? 4 * kTE : 0));               // [kTE][YS]
+    TA* tab_w = (TA*)(gcon + gcon_doubles(kGtoConst, L1, n));               // [4 waves][16][4] weights (table)
+    int32_t* tab_row = (int32_t*)(tab_w + (RBF == kRbfTable ? 4 * kTE : 0)); // [4 waves][16] row offsets (table)
+    TY* Ys = (TY*)(tab_row + (RBF == kRbfTable ? kTE : 0));                 // [kTE][YS]
     TA* Rs = (TA*)(Ys + kTE * YS);                    // [kTE][RS]; later the combine / beta stage
 
     const int c = blockIdx.x, tid = threadIdx.x;
@@ -162,6 +165,10 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const double* g_gamma = gcon;
     const double* g_pref = gcon + L1 * n;
     const double* g_beta = gcon + 2 * L1 * n;
+    // table phase: this lane's (l, n) channel of the first 64; grid constants
+    const int L1n = L1 * n, ch_l = lane / n, ch_n = lane - (lane / n) * n;
+    const double tab_inv_h = RBF == kRbfTable ? b.rad_intervals / b.rad_rmax : 0.0;
+    const TA tab_h = RBF == kRbfTable ? (TA)(b.rad_rmax / b.rad_intervals) : TA(0);
     __syncthreads();
 
     for (int s0 = 0; s0 < S; s0 += 4) {
@@ -215,74 +222,86 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                     const int spa = atom_sp[a];
                     sp = ((create_path && !(r > 1e-12)) || spa < s0 || spa >= s1) ? -1 : spa;
                 }
-                uint64_t msk[4];
+                // compact the tile's counted edges by species: position p = start of the
+                // species + rank of the edge among that species' edges (ballot + mbcnt), the
+                // same in every wavefront; LDS rows are indexed by p, so the loops below run
+                // over contiguous uniform ranges instead of walking lane masks
+                int beg[4], scnt[4], pidx = -1, nv = 0;
 #pragma unroll
-                for (int a4 = 0; a4 < 4; ++a4) msk[a4] = __ballot(sp == s0 + a4);
+                for (int a4 = 0; a4 < 4; ++a4) {
+                    const uint64_t m = __ballot(sp == s0 + a4);
+                    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    beg[a4] = nv;
+                    scnt[a4] = __popcll(m);
+                    if (sp == s0 + a4) pidx = nv + rank;
+                    nv += scnt[a4];
+                }
                 if (RBF == kRbfTable) {
                     // radial table: lanes = (l, n) channels of one edge, so each edge reads its
-                    // two node rows with coalesced loads; wavefront q takes edges e = q (mod 4),
-                    // four edges per round so that their sixteen row loads are in flight together.
-                    // The edge's own lane first stores its weighted Hermite factors and row offset.
-                    uint64_t todo = __ballot(sp >= 0) & (0x1111111111111111ull << q);
+                    // two node rows with coalesced 8-byte (value, slope) loads; wavefront q takes
+                    // positions p = q + 4k (k < 16), four per round so that their loads are in
+                    // flight together.  The edge's own lane first stores its weighted Hermite
+                    // factors and row offset in the wavefront's slots k.
                     const int G = b.rad_intervals, NS = L1 * 2 * n;
-                    if (sp >= 0 && (lane & 3) == q) {
-                        const double xr = r * ((double)G / b.rad_rmax);
+                    TA* wq = tab_w + q * 4 * 16;
+                    int32_t* rq = tab_row + q * 16;
+                    if (pidx >= 0 && (pidx & 3) == q) {
+                        const double xr = r * tab_inv_h;
                         int i = (int)xr;
                         i = i < G - 1 ? i : G - 1;
-                        const TA t = (TA)(xr - i), hh = (TA)(b.rad_rmax / G), tw = (TA)w;
+                        const TA t = (TA)(xr - i), tw = (TA)w;
                         const TA t2 = t * t, t3 = t2 * t;
-                        TA* hw = tab_w + lane * 4;
+                        TA* hw = wq + (pidx >> 2) * 4;
                         hw[0] = tw * (TA(2) * t3 - TA(3) * t2 + TA(1));
-                        hw[1] = tw * hh * (t3 - TA(2) * t2 + t);
+                        hw[1] = tw * tab_h * (t3 - TA(2) * t2 + t);
                         hw[2] = tw * (TA(3) * t2 - TA(2) * t3);
-                        hw[3] = tw * hh * (t3 - t2);
-                        tab_row[lane] = ((int64_t)(cslot * S + sp) * (G + 1) + i) * NS;
+                        hw[3] = tw * tab_h * (t3 - t2);
+                        rq[pidx >> 2] = ((cslot * S + sp) * (G + 1) + i) * NS;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    while (todo) {
-                        int ev[4];
-                        int ne = 0;
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            ev[j] = todo ? __builtin_ctzll(todo) : -1;
-                            if (todo) {
-                                todo &= todo - 1;
-                                ++ne;
+                    const int nk = (nv - q + 3) >> 2;       // this wavefront's positions
+                    for (int k0 = 0; k0 < nk; k0 += 4) {
+                        const int ne = nk - k0 < 4 ? nk - k0 : 4;
+                        const int4 rows = *(const int4*)(rq + k0);
+                        const int rw[4] = {rows.x, ne > 1 ? rows.y : rows.x, ne > 2 ? rows.z : rows.x,
+                                           ne > 3 ? rows.w : rows.x};
+                        for (int ch0 = 0; ch0 < L1n; ch0 += 64) {
+                            const bool on = ch0 + lane < L1n;
+                            int ll = ch_l, nn = ch_n;
+                            if (ch0 > 0) {
+                                ll = (ch0 + lane) / n;
+                                nn = ch0 + lane - ll * n;
                             }
-                        }
-                        for (int ch0 = 0; ch0 < L1 * n; ch0 += 64) {
-                            const int ch = ch0 + lane;
-                            const bool on = ch < L1 * n;
-                            const int ll = on ? ch / n : 0, nn = on ? ch - ll * n : 0;
-                            const int coff = ll * 2 * n + nn;
-                            float v[4][4];
+                            const int coff = on ? (ll * n + nn) * 2 : 0;
+                            float2 v0[4], v1[4];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
-                                const int e = ev[j] < 0 ? ev[0] : ev[j];
-                                const float* a0 = b.rad_table + tab_row[e] + coff;
-                                v[j][0] = a0[0];
-                                v[j][1] = a0[n];
-                                v[j][2] = a0[NS];
-                                v[j][3] = a0[NS + n];
+                                const float* a0 = b.rad_table + (uint32_t)(rw[j] + coff);
+                                v0[j] = *(const float2*)a0;
+                                v1[j] = *(const float2*)(a0 + NS);
                             }
+                            // slots j >= ne (k0 + j < 16, so position < 64) fill rows that phase B
+                            // never reads: no per-slot branch
+                            if (on) {
 #pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                if (j >= ne || !on) continue;
-                                const TA* hw = tab_w + ev[j] * 4;
-                                Rs[ev[j] * RS + ll * nP + nn] = hw[0] * (TA)v[j][0] + hw[1] * (TA)v[j][1] +
-                                                                hw[2] * (TA)v[j][2] + hw[3] * (TA)v[j][3];
+                                for (int j = 0; j < 4; ++j) {
+                                    const typename V2<TA>::type hw01 = *(const typename V2<TA>::type*)(wq + (k0 + j) * 4);
+                                    const typename V2<TA>::type hw23 = *(const typename V2<TA>::type*)(wq + (k0 + j) * 4 + 2);
+                                    Rs[(q + 4 * (k0 + j)) * RS + ll * nP + nn] = hw01.x * (TA)v0[j].x + hw01.y * (TA)v0[j].y +
+                                                                                 hw23.x * (TA)v1[j].x + hw23.y * (TA)v1[j].y;
+                                }
                             }
                         }
                     }
                 }
-                if (sp >= 0) {
-                    TY* yrow = Ys + lane * YS;
+                if (pidx >= 0) {
+                    TY* yrow = Ys + pidx * YS;
                     solid_harmonics_stripe<LMAX, TY>((TY)rx, (TY)ry, (TY)rz, q, [&](int ll, int m, TY v) {
                         yrow[4 * group_start(ll) + ll + m] = v;
                     });
-                    TA* rrow = Rs + lane * RS;
+                    TA* rrow = Rs + pidx * RS;
                     if (RBF == kRbfTable) {
                         // radial values written above
                     } else if (RBF == CUSOAP_RBF_POLY) {
@@ -333,16 +352,12 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                     }
                 }
                 __syncthreads();
-                // ---- B: wavefront wv sums its edges e = wv (mod 4), species by species
+                // ---- B: wavefront q sums the positions p = q (mod 4) of each species range
                 if (valid) {
-                    const uint64_t mine = 0x1111111111111111ull << wv;
 #pragma unroll
                     for (int a4 = 0; a4 < 4; ++a4) {
                         if (s0 + a4 >= s1) continue;
-                        uint64_t m = msk[a4] & mine;
-                        while (m) {
-                            const int e = __builtin_ctzll(m);
-                            m &= m - 1;
+                        auto fma_edge = [&](int e) {
                             const typename V2<TA>::type rv = *(const typename V2<TA>::type*)(Rs + e * RS + roff);
                             const typename V2<TY>::type y01 = *(const typename V2<TY>::type*)(Ys + e * YS + yoff);
                             const typename V2<TY>::type y23 = *(const typename V2<TY>::type*)(Ys + e * YS + yoff + 2);
@@ -352,7 +367,15 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                                 acc[a4][0][j] += rv.x * yv[j];
                                 acc[a4][1][j] += rv.y * yv[j];
                             }
+                        };
+                        // two positions per step so that their LDS reads are in flight together
+                        const int pe = beg[a4] + scnt[a4];
+                        int pp = beg[a4] + ((q - beg[a4]) & 3);
+                        for (; pp + 4 < pe; pp += 8) {
+                            fma_edge(pp);
+                            fma_edge(pp + 4);
                         }
+                        if (pp < pe) fma_edge(pp);
                     }
                 }
                 __syncthreads();
@@ -429,7 +452,7 @@ int launch_coeff(const cusoap_basis_t& b, const double* pos, int n_atoms, const 
     const size_t rs_bytes = sizeof(TA) * (size_t)kTE * r_stride(LMAX, n);
     const size_t part = sizeof(TA) * (size_t)std::max(b.n_species, 4) * n * L2;   // combine / beta stage (aliases Rs)
     const size_t gcon = sizeof(double) * (size_t)gcon_doubles(RBF == CUSOAP_RBF_GTO, L1, n);
-    const size_t tab = RBF == kRbfTable ? (sizeof(int64_t) + 4 * sizeof(TA)) * (size_t)kTE : 0;
+    const size_t tab = RBF == kRbfTable ? (sizeof(int32_t) + 4 * sizeof(TA)) * (size_t)kTE : 0;
     const size_t smem = gcon + tab + ys_bytes + (part > rs_bytes ? part : rs_bytes);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: LDS tile too large");
     auto kern = coeff_kernel<LMAX, PREC, RBF>;
@@ -476,6 +499,8 @@ int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int 
     if (create_path && !self_sp) return fail(CUSOAP_ERR_ARG, "coefficients: create path needs center_self_species");
     if (b->rad_table && (!self_sp || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
         return fail(CUSOAP_ERR_ARG, "coefficients: radial table needs center_self_species and a valid grid");
+    if (b->rad_table && (double)(b->n_species + 1) * b->n_species * (b->rad_intervals + 1.0) * (b->l_max + 1) * 2 * b->n_max >= 2147483647.0)
+        return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: radial table larger than 2^31 values");
     if (n_centers == 0) return CUSOAP_OK;
 #define CUSOAP_CASE(L) \
     case L: return dispatch_prec<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st);

@@ -245,6 +245,82 @@ __device__ __forceinline__ void solid_harmonics_lmajor(T x, T y, T z, OnL&& on_l
     }
 }
 
+// Adjoint of the solid harmonics S_lm = K_lm Q_lm(z, r^2) Phi_m(x, y), with the
+// normalisation K_lm folded into the caller's coefficients.  For each degree l,
+// on_l(l) once, then h(lm, h0, h1) returns the caller's two coefficients of lm
+// (h0 multiplies grad S, h1 multiplies x S).  Accumulates
+//   sum_lm [h1 x S_lm + h0 grad S_lm] / K_lm = x P + (X, Y, Z)
+// using grad Q = 2 (x, y, z) dQ/dr^2 + z-hat dQ/dz and
+// grad (x+iy)^m = m (x+iy)^(m-1) (1, i): per (l, |m|) pair the two orders share
+// Q, dQ/dz, dQ/dr^2, so the gradient costs a few fused products per lm instead
+// of forming the three gradient components of every harmonic.
+template <int LMAX, typename T, typename OnL, typename H>
+__device__ __forceinline__ void solid_harmonics_adjoint(T x, T y, T z, OnL&& on_l, H&& h, T& P, T& X, T& Y, T& Z) {
+    const T r2 = (x * x + y * y) + z * z;
+    T cm[LMAX + 1], sm[LMAX + 1];
+    cm[0] = T(1);
+    sm[0] = T(0);
+#pragma unroll
+    for (int m = 1; m <= LMAX; ++m) {
+        cm[m] = x * cm[m - 1] - y * sm[m - 1];
+        sm[m] = x * sm[m - 1] + y * cm[m - 1];
+    }
+    T q1[LMAX + 1], q1z[LMAX + 1], q1r[LMAX + 1], q2[LMAX + 1], q2z[LMAX + 1], q2r[LMAX + 1];
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) q1[m] = q1z[m] = q1r[m] = q2[m] = q2z[m] = q2r[m] = T(0);
+#pragma unroll
+    for (int l = 0; l <= LMAX; ++l) {
+        T q[LMAX + 1], qz[LMAX + 1], qr[LMAX + 1];
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            if (m == l) {
+                q[m] = T(kDoubleFact[m]);
+                qz[m] = T(0);
+                qr[m] = T(0);
+            } else if (m == l - 1) {
+                q[m] = T(2 * m + 1) * z * q1[m];
+                qz[m] = T(2 * m + 1) * q1[m];
+                qr[m] = T(0);
+            } else {
+                const T a = T(2 * l - 1), bb = T(l + m - 1), inv = T(1.0 / (l - m));
+                q[m] = (a * z * q1[m] - bb * r2 * q2[m]) * inv;
+                qz[m] = (a * (q1[m] + z * q1z[m]) - bb * r2 * q2z[m]) * inv;
+                qr[m] = (a * z * q1r[m] - bb * (q2[m] + r2 * q2r[m])) * inv;
+            }
+        }
+        const int c0 = l * l + l;
+        on_l(l);
+        {
+            T h0, h1;
+            h(c0, h0, h1);
+            P += q[0] * h1 + (T(2) * qr[0]) * h0;
+            Z += qz[0] * h0;
+        }
+#pragma unroll
+        for (int m = 1; m <= l; ++m) {
+            T h0p, h1p, h0n, h1n;
+            h(c0 + m, h0p, h1p);                       // cos-type (x+iy)^m real part
+            h(c0 - m, h0n, h1n);                       // sin-type imaginary part
+            const T q2r_ = T(2) * qr[m];
+            const T tp = q[m] * h1p + q2r_ * h0p, tn = q[m] * h1n + q2r_ * h0n;
+            P += cm[m] * tp + sm[m] * tn;
+            Z += qz[m] * (cm[m] * h0p + sm[m] * h0n);
+            const T u = T(m) * q[m];
+            X += u * (cm[m - 1] * h0p + sm[m - 1] * h0n);
+            Y += u * (cm[m - 1] * h0n - sm[m - 1] * h0p);
+        }
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            q2[m] = q1[m];
+            q2z[m] = q1z[m];
+            q2r[m] = q1r[m];
+            q1[m] = q[m];
+            q1z[m] = qz[m];
+            q1r[m] = qr[m];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // weighting (w(r) and dw/dr; w0 and zero slope below 1e-12)
 // ---------------------------------------------------------------------------

@@ -522,8 +522,9 @@ __global__ void node_inputs_kernel(int n_nodes, double h, double r_fit, float* _
     x[i] = (float)(2.0 * r / r_fit - 1.0);
 }
 
-// table ((S+1) S slots, G+1 nodes, L1, 2, n): slot = center_slot * S + neighbour species,
-// center_slot = S for centers that are not atoms (always the exact function)
+// table ((S+1) S slots, G+1 nodes, L1, n, 2): (B, dB/dr) pairs, so a kernel reads
+// one channel's node with one 8-byte load; slot = center_slot * S + neighbour
+// species, center_slot = S for centers that are not atoms (always the exact function)
 __global__ void table_kernel(cusoap_basis_t b, const float* __restrict__ y, const float* __restrict__ dy,
                              const int32_t* __restrict__ use_pair, const float* __restrict__ scale, double r_fit,
                              int G, double h, float* __restrict__ table) {
@@ -549,8 +550,8 @@ __global__ void table_kernel(cusoap_basis_t b, const float* __restrict__ y, cons
             const size_t k = ((size_t)pair * n_nodes + node) * O + j;
             const double sj = (double)scale[(size_t)pair * O + j];
             const double g = sj * (double)y[k], gp = sj * (2.0 / r_fit) * (double)dy[k];
-            dst[nn] = (float)(g / ul);
-            dst[n + nn] = (float)(gp / ul - l * g / (r_fit * ul * u));
+            dst[2 * nn] = (float)(g / ul);
+            dst[2 * nn + 1] = (float)(gp / ul - l * g / (r_fit * ul * u));
         }
         return;
     }
@@ -564,8 +565,8 @@ __global__ void table_kernel(cusoap_basis_t b, const float* __restrict__ y, cons
             v += beta[k] * q[k];
             vh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q[k];
         }
-        dst[nn] = (float)v;
-        dst[n + nn] = (float)(r * vh);
+        dst[2 * nn] = (float)v;
+        dst[2 * nn + 1] = (float)(r * vh);
     }
 }
 

@@ -263,12 +263,14 @@ def exact_radial(soap, r: np.ndarray):
 
 def radial_table(soap, n_intervals: int = 1024, r_max: Optional[float] = None,
                  model: Optional[RadialSurrogate] = None) -> torch.Tensor:
-    """(S+1) S x (n_intervals+1) x (l_max+1) x 2 x n_max float32 table on the device."""
+    """(S+1) S x (n_intervals+1) x (l_max+1) x n_max x 2 float32 table on the device.
+
+    The last axis holds (B_nl(r_i), dB_nl/dr(r_i)) at node r_i."""
     lib = _lib.load()
     b = soap._basis_exact()
     r_max = float(soap._cutoff if r_max is None else r_max)
     S, L1, n = soap.n_species, soap._l_max + 1, soap._n_max
-    table = torch.empty(((S + 1) * S, n_intervals + 1, L1, 2, n), dtype=torch.float32, device=soap.device)
+    table = torch.empty(((S + 1) * S, n_intervals + 1, L1, n, 2), dtype=torch.float32, device=soap.device)
     if model is None:
         _lib.check(lib.cusoap_radial_table(C.byref(b), None, HIDDEN, None, None, 1.0, n_intervals, r_max, None, 0,
                                            table.data_ptr(), soap._stream()), "radial_table")

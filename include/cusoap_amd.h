@@ -71,8 +71,8 @@ typedef struct cusoap_basis {
     const double* self_l0;      /* (n_max)           l=0 self coefficient (incl. w0, Y00) */
     /* Tabulated radial model (the learned radial surrogate, see cusoap_radial_table),
      * float32 device array or NULL for the exact radial functions.  Layout
-     * ((S+1) S slots, rad_intervals+1 nodes, l_max+1, 2, n_max): B_nl(r_i) then
-     * dB_nl/dr(r_i) at r_i = i rad_rmax / rad_intervals; slot = center_slot * S +
+     * ((S+1) S slots, rad_intervals+1 nodes, l_max+1, n_max, 2): the pair
+     * (B_nl(r_i), dB_nl/dr(r_i)) at r_i = i rad_rmax / rad_intervals; slot = center_slot * S +
      * neighbour species, center_slot = the center atom's species, or S for a
      * center that is not an atom.  Kernels interpolate cubic Hermite in r. */
     const float* rad_table;
