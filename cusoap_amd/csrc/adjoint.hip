@@ -18,14 +18,17 @@
 // normalisation K_lm.  With one descriptor gradient for all centers (the
 // linear-readout case) W_l is built once and a grid-stride kernel forms G from
 // it; otherwise one workgroup per center builds its own W_l.
-// Kernel 2 (GTO; 256 threads per center, thread per edge) walks the solid
+// Kernel 2 (GTO; thread per edge, 256 threads per group of up to 8 centers whose
+// edges form one contiguous range, so lanes stay busy across centers) walks the solid
 // harmonics l-major, builds the radial vectors of degree l once per degree and
 // contracts: H0 = sum_k rad0_k M[s][lm][k], H1 = sum_k rad1_k M[s][lm][k]; the
 // gradient sum_lm [r H1 Y_lm + H0 grad Y_lm] is accumulated as r P + (X, Y, Z)
 // from the shared factors of each (l, |m|) pair (solid_harmonics_adjoint), and
 // the 3-vector is scattered with float64 atomics.  The polynomial basis uses a
 // float64 tile kernel.
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -245,6 +248,16 @@ __device__ __forceinline__ double t_exp2<double>(double v) { return exp(v); }
 // with a compile-time n_max (NN > 0), in registers, so that each lm needs only
 // the 16-byte reads of its M row.
 constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
+// centers per workgroup of the edge kernel (at most): the ALU-bound forms keep all
+// lanes busy with groups of 8; the register-table form is bound by the latency of
+// its table reads and measured fastest with one center per workgroup
+constexpr int kVjpGroup = 8;
+template <int NN>
+constexpr int vjp_group() { return NN > 0 ? 1 : kVjpGroup; }
+// the register-table form also gains from 4 wavefronts per SIMD (a few spilled
+// registers cost less than the latency the extra wavefronts hide)
+template <int NN>
+constexpr int vjp_min_waves() { return NN > 0 ? 4 : 1; }
 
 // M row of one lm into registers (16-byte reads when the row allows)
 template <typename T, int N>
@@ -262,7 +275,7 @@ __device__ __forceinline__ void load_row(const T* p, T (&v)[N]) {
 }
 
 template <int LMAX, typename TM, int RAD, int NN>
-__global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
+__global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
                                                       const int32_t* __restrict__ atom_sp,
                                                       const double* __restrict__ shifts,
                                                       const double* __restrict__ centers,
@@ -270,219 +283,246 @@ __global__ __launch_bounds__(256) void vjp_gto_kernel(cusoap_basis_t b, const do
                                                       const int32_t* __restrict__ center_species,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr, const TM* __restrict__ M,
-                                                      double* __restrict__ grad) {
+                                                      int n_centers, int ngrp, double* __restrict__ grad) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     constexpr int NT = 256;
     constexpr bool MIXED = RAD == kRadMixed;
     constexpr bool FIXN = NN > 0;
     constexpr int NR = FIXN ? NN : 1;
     const int n = FIXN ? NN : b.n_max, S = b.n_species;
-    int cslot = S;
-    if (RAD == kRadTable && center_species[blockIdx.x] >= 0) cslot = center_species[blockIdx.x];
     extern __shared__ __align__(16) unsigned char smem[];
     double* qd = (double*)smem;                             // [n][NT]   (MIXED, runtime n) primitives
     TM* rad0 = (TM*)(qd + (MIXED && !FIXN ? n * NT : 0));   // [n][NT]   (runtime n)
     TM* rad1 = rad0 + (FIXN ? 0 : n * NT);                  // [n][NT]   (runtime n)
-    TM* Ms = rad1 + (FIXN ? 0 : n * NT);                    // [S][L2][n]
-    __shared__ double red[3][4];
-    const int c = blockIdx.x, tid = threadIdx.x;
+    TM* Ms = rad1 + (FIXN ? 0 : n * NT);                    // [ngrp][S][L2][n]
+    __shared__ int64_t goff[kVjpGroup + 1];                 // the group's edge offsets
+    __shared__ double gcen[kVjpGroup][3];
+    __shared__ int gslot[kVjpGroup];                        // table slot of each center
+    __shared__ double red[kVjpGroup][3];                    // center-attach sums
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int c0 = blockIdx.x * ngrp, ng = n_centers - c0 < ngrp ? n_centers - c0 : ngrp;
     const int per = S * n * L2;
-    for (int i = tid; i < per; i += NT) Ms[i] = M[(size_t)c * per + i];
+    if (tid <= ng) goff[tid] = offsets[c0 + tid];
+    if (tid < ng) {
+        for (int d = 0; d < 3; ++d) {
+            gcen[tid][d] = centers[3 * (size_t)(c0 + tid) + d];
+            red[tid][d] = 0.0;
+        }
+        gslot[tid] = (RAD == kRadTable && center_species[c0 + tid] >= 0) ? center_species[c0 + tid] : S;
+    }
+    {
+        // the group's coefficient rows: 16-byte loads, four in flight per thread
+        const TM* src = M + (size_t)c0 * per;
+        const int tot = ng * per;
+        constexpr int V = 16 / sizeof(TM);
+        const int nv = (((size_t)c0 * per) % V == 0) ? tot / V : 0;   // rows of whole centers stay 16-byte aligned
+        using Vec = typename std::conditional<sizeof(TM) == 4, float4, double2>::type;
+        const Vec* s4 = (const Vec*)src;
+        Vec* d4 = (Vec*)Ms;
+        int i = tid;
+        for (; i + 3 * NT < nv; i += 4 * NT) {
+            const Vec a0 = s4[i], a1 = s4[i + NT], a2 = s4[i + 2 * NT], a3 = s4[i + 3 * NT];
+            d4[i] = a0;
+            d4[i + NT] = a1;
+            d4[i + 2 * NT] = a2;
+            d4[i + 3 * NT] = a3;
+        }
+        for (; i < nv; i += NT) d4[i] = s4[i];
+        for (int j = nv * V + tid; j < tot; j += NT) Ms[j] = src[j];
+    }
     __syncthreads();
-    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
-    const int64_t e0 = offsets[c], e1 = offsets[c + 1];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const int64_t E0 = goff[0], E1 = goff[ng];
     TM* r0c = rad0 + tid;
     TM* r1c = rad1 + tid;
     double* qc = qd + tid;
-    for (int64_t e = e0 + tid; e < e1; e += NT) {
-        const int ext = nbr[e];
-        const int img = ext / n_atoms, a = ext - img * n_atoms;
-        const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
-        const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
-        const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
-        const double r2 = (rx * rx + ry * ry) + rz * rz;
-        const double r = sqrt(r2);
-        double w = 1.0, dw = 0.0;
-        if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
-        const double dwr = r > 0.0 ? dw / r : 0.0;
-        const TM* Mrow = Ms + (size_t)atom_sp[a] * L2 * n;
-        const TM trx = (TM)rx, tryy = (TM)ry, trz = (TM)rz;
+    // the group's edges as one flat range (lanes stay busy across center boundaries)
+    for (int64_t base = E0; base < E1; base += NT) {
+        const int64_t e = base + tid;
+        int ci = -1;
         TM gx = TM(0), gy = TM(0), gz = TM(0);
-        // table interpolation weights: value (v*) and (1/r) d/dr (c*)
-        const float* tb = nullptr;
-        TM v00 = TM(0), v10 = TM(0), v01 = TM(0), v11 = TM(0), c00 = TM(0), c10 = TM(0), c01 = TM(0), c11 = TM(0);
-        if (RAD == kRadTable) {
-            const int G = b.rad_intervals;
-            const double hd = b.rad_rmax / G, xr = r / hd;
-            int i = (int)xr;
-            i = i < G - 1 ? i : G - 1;
-            const double t = xr - i, t2 = t * t, t3 = t2 * t;
-            v00 = (TM)(2.0 * t3 - 3.0 * t2 + 1.0);
-            v01 = (TM)(3.0 * t2 - 2.0 * t3);
-            v10 = (TM)(hd * (t3 - 2.0 * t2 + t));
-            v11 = (TM)(hd * (t3 - t2));
-            if (i == 0) {
-                // the node slope at r = 0 is zero, so P'(r)/r is a polynomial on the first interval
-                c00 = (TM)(6.0 * (t - 1.0) / (hd * hd));
-                c01 = -c00;
-                c11 = (TM)((3.0 * t - 2.0) / hd);
-            } else {
-                const double ir = 1.0 / r;
-                c00 = (TM)((6.0 * t2 - 6.0 * t) / hd * ir);
-                c10 = (TM)((3.0 * t2 - 4.0 * t + 1.0) * ir);
-                c01 = (TM)((6.0 * t - 6.0 * t2) / hd * ir);
-                c11 = (TM)((3.0 * t2 - 2.0 * t) * ir);
-            }
-            tb = b.rad_table + ((size_t)(cslot * S + atom_sp[a]) * (G + 1) + i) * (L1 * 2 * n);
-            // fold the weighting in: rad0 = w P(r), rad1 = w P'(r)/r + (w'/r) P(r)
-            const TM tw = (TM)w, tdwr = (TM)dwr;
-            c00 = tw * c00 + tdwr * v00;
-            c10 = tw * c10 + tdwr * v10;
-            c01 = tw * c01 + tdwr * v01;
-            c11 = tw * c11 + tdwr * v11;
-            v00 *= tw;
-            v10 *= tw;
-            v01 *= tw;
-            v11 *= tw;
-        }
-        TM rr0[NR], rr1[NR];                                 // radial vectors of degree l (FIXN)
-        auto on_l = [&](int l) {
-            if constexpr (FIXN) {
+        if (e < E1) {
+            ci = 0;
+            while (ci + 1 < ng && goff[ci + 1] <= e) ++ci;
+            const double cx = gcen[ci][0], cy = gcen[ci][1], cz = gcen[ci][2];
+            const int cslot = gslot[ci];
+            const int ext = nbr[e];
+            const int img = ext / n_atoms, a = ext - img * n_atoms;
+            const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
+            const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
+            const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
+            const double r2 = (rx * rx + ry * ry) + rz * rz;
+            const double r = sqrt(r2);
+            double w = 1.0, dw = 0.0;
+            if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
+            const double dwr = r > 0.0 ? dw / r : 0.0;
+            const TM* Mrow = Ms + (size_t)ci * per + (size_t)atom_sp[a] * L2 * n;
+            const TM trx = (TM)rx, tryy = (TM)ry, trz = (TM)rz;
+            // table interpolation weights: value (v*) and (1/r) d/dr (c*)
+            const float* tb = nullptr;
+            TM v00 = TM(0), v10 = TM(0), v01 = TM(0), v11 = TM(0), c00 = TM(0), c10 = TM(0), c01 = TM(0), c11 = TM(0);
+            if (RAD == kRadTable) {
+                const int G = b.rad_intervals;
+                const double hd = b.rad_rmax / G, xr = r / hd;
+                int i = (int)xr;
+                i = i < G - 1 ? i : G - 1;
+                const double t = xr - i, t2 = t * t, t3 = t2 * t;
+                v00 = (TM)(2.0 * t3 - 3.0 * t2 + 1.0);
+                v01 = (TM)(3.0 * t2 - 2.0 * t3);
+                v10 = (TM)(hd * (t3 - 2.0 * t2 + t));
+                v11 = (TM)(hd * (t3 - t2));
+                if (i == 0) {
+                    // the node slope at r = 0 is zero, so P'(r)/r is a polynomial on the first interval
+                    c00 = (TM)(6.0 * (t - 1.0) / (hd * hd));
+                    c01 = -c00;
+                    c11 = (TM)((3.0 * t - 2.0) / hd);
+                } else {
+                    const double ir = 1.0 / r;
+                    c00 = (TM)((6.0 * t2 - 6.0 * t) / hd * ir);
+                    c10 = (TM)((3.0 * t2 - 4.0 * t + 1.0) * ir);
+                    c01 = (TM)((6.0 * t - 6.0 * t2) / hd * ir);
+                    c11 = (TM)((3.0 * t2 - 2.0 * t) * ir);
+                }
+                tb = b.rad_table + ((size_t)(cslot * S + atom_sp[a]) * (G + 1) + i) * (L1 * 2 * n);
+                // fold the weighting in: rad0 = w P(r), rad1 = w P'(r)/r + (w'/r) P(r)
                 const TM tw = (TM)w, tdwr = (TM)dwr;
+                c00 = tw * c00 + tdwr * v00;
+                c10 = tw * c10 + tdwr * v10;
+                c01 = tw * c01 + tdwr * v01;
+                c11 = tw * c11 + tdwr * v11;
+                v00 *= tw;
+                v10 *= tw;
+                v01 *= tw;
+                v11 *= tw;
+            }
+            TM rr0[NR], rr1[NR];                                 // radial vectors of degree l (FIXN)
+            auto on_l = [&](int l) {
+                if constexpr (FIXN) {
+                    const TM tw = (TM)w, tdwr = (TM)dwr;
+                    if (RAD == kRadTable) {
+                        const float* p0 = tb + l * 2 * NN;
+                        const float* p1 = p0 + L1 * 2 * NN;
+    #pragma unroll
+                        for (int k = 0; k < NN; ++k) {
+                            const TM b0 = (TM)p0[2 * k], d0 = (TM)p0[2 * k + 1], b1 = (TM)p1[2 * k], d1 = (TM)p1[2 * k + 1];
+                            rr0[k] = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
+                            rr1[k] = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
+                        }
+                    } else if (MIXED) {
+                        double qv[NN];
+    #pragma unroll
+                        for (int k = 0; k < NN; ++k) qv[k] = b.gto_pref[l * NN + k] * exp(-b.gto_gamma[l * NN + k] * r2);
+    #pragma unroll
+                        for (int nn = 0; nn < NN; ++nn) {
+                            const double* beta = b.gto_beta + ((size_t)l * NN + nn) * NN;
+                            double B = 0.0, Bh = 0.0;
+    #pragma unroll
+                            for (int k = 0; k < NN; ++k) {
+                                B += beta[k] * qv[k];
+                                Bh += beta[k] * (-2.0 * b.gto_gamma[l * NN + k]) * qv[k];
+                            }
+                            rr0[nn] = (TM)(w * B);
+                            rr1[nn] = (TM)(w * Bh + dwr * B);
+                        }
+                    } else {
+    #pragma unroll
+                        for (int k = 0; k < NN; ++k) {
+                            const TM gam = (TM)b.gto_gamma[l * NN + k];
+                            const TM q = (TM)b.gto_pref[l * NN + k] * t_exp2<TM>(-gam * (TM)r2);
+                            rr0[k] = tw * q;
+                            rr1[k] = (tw * (TM(-2) * gam) + tdwr) * q;
+                        }
+                    }
+                    return;
+                }
                 if (RAD == kRadTable) {
-                    const float* p0 = tb + l * 2 * NN;
-                    const float* p1 = p0 + L1 * 2 * NN;
-#pragma unroll
-                    for (int k = 0; k < NN; ++k) {
-                        const TM b0 = (TM)p0[2 * k], d0 = (TM)p0[2 * k + 1], b1 = (TM)p1[2 * k], d1 = (TM)p1[2 * k + 1];
-                        rr0[k] = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
-                        rr1[k] = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
+                    const float* p0 = tb + l * 2 * n;
+                    const float* p1 = p0 + L1 * 2 * n;
+                    auto emit = [&](int nn, float b0, float d0, float b1, float d1) {
+                        r0c[nn * NT] = v00 * (TM)b0 + v10 * (TM)d0 + v01 * (TM)b1 + v11 * (TM)d1;
+                        r1c[nn * NT] = c00 * (TM)b0 + c10 * (TM)d0 + c01 * (TM)b1 + c11 * (TM)d1;
+                    };
+                    if ((n & 1) == 0) {
+                        // node rows start on 16-byte boundaries: two 16-byte loads per 2 functions
+                        for (int k2 = 0; k2 < n; k2 += 2) {
+                            const float4 a = *(const float4*)(p0 + 2 * k2), c = *(const float4*)(p1 + 2 * k2);
+                            emit(k2, a.x, a.y, c.x, c.y);
+                            emit(k2 + 1, a.z, a.w, c.z, c.w);
+                        }
+                    } else {
+                        for (int nn = 0; nn < n; ++nn) emit(nn, p0[2 * nn], p0[2 * nn + 1], p1[2 * nn], p1[2 * nn + 1]);
                     }
                 } else if (MIXED) {
-                    double qv[NN];
-#pragma unroll
-                    for (int k = 0; k < NN; ++k) qv[k] = b.gto_pref[l * NN + k] * exp(-b.gto_gamma[l * NN + k] * r2);
-#pragma unroll
-                    for (int nn = 0; nn < NN; ++nn) {
-                        const double* beta = b.gto_beta + ((size_t)l * NN + nn) * NN;
+                    for (int k = 0; k < n; ++k) qc[k * NT] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);
+                    for (int nn = 0; nn < n; ++nn) {
+                        const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
                         double B = 0.0, Bh = 0.0;
-#pragma unroll
-                        for (int k = 0; k < NN; ++k) {
-                            B += beta[k] * qv[k];
-                            Bh += beta[k] * (-2.0 * b.gto_gamma[l * NN + k]) * qv[k];
+                        for (int k = 0; k < n; ++k) {
+                            const double q = qc[k * NT];
+                            B += beta[k] * q;
+                            Bh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q;
                         }
-                        rr0[nn] = (TM)(w * B);
-                        rr1[nn] = (TM)(w * Bh + dwr * B);
+                        r0c[nn * NT] = (TM)(w * B);
+                        r1c[nn * NT] = (TM)(w * Bh + dwr * B);
                     }
                 } else {
-#pragma unroll
-                    for (int k = 0; k < NN; ++k) {
-                        const TM gam = (TM)b.gto_gamma[l * NN + k];
-                        const TM q = (TM)b.gto_pref[l * NN + k] * t_exp2<TM>(-gam * (TM)r2);
-                        rr0[k] = tw * q;
-                        rr1[k] = (tw * (TM(-2) * gam) + tdwr) * q;
-                    }
-                }
-                return;
-            }
-            if (RAD == kRadTable) {
-                const float* p0 = tb + l * 2 * n;
-                const float* p1 = p0 + L1 * 2 * n;
-                auto emit = [&](int nn, float b0, float d0, float b1, float d1) {
-                    r0c[nn * NT] = v00 * (TM)b0 + v10 * (TM)d0 + v01 * (TM)b1 + v11 * (TM)d1;
-                    r1c[nn * NT] = c00 * (TM)b0 + c10 * (TM)d0 + c01 * (TM)b1 + c11 * (TM)d1;
-                };
-                if ((n & 1) == 0) {
-                    // node rows start on 16-byte boundaries: two 16-byte loads per 2 functions
-                    for (int k2 = 0; k2 < n; k2 += 2) {
-                        const float4 a = *(const float4*)(p0 + 2 * k2), c = *(const float4*)(p1 + 2 * k2);
-                        emit(k2, a.x, a.y, c.x, c.y);
-                        emit(k2 + 1, a.z, a.w, c.z, c.w);
-                    }
-                } else {
-                    for (int nn = 0; nn < n; ++nn) emit(nn, p0[2 * nn], p0[2 * nn + 1], p1[2 * nn], p1[2 * nn + 1]);
-                }
-            } else if (MIXED) {
-                for (int k = 0; k < n; ++k) qc[k * NT] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);
-                for (int nn = 0; nn < n; ++nn) {
-                    const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
-                    double B = 0.0, Bh = 0.0;
                     for (int k = 0; k < n; ++k) {
-                        const double q = qc[k * NT];
-                        B += beta[k] * q;
-                        Bh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q;
+                        const TM gam = (TM)b.gto_gamma[l * n + k];
+                        const TM q = (TM)b.gto_pref[l * n + k] * t_exp2<TM>(-gam * (TM)r2);
+                        r0c[k * NT] = (TM)w * q;
+                        r1c[k * NT] = ((TM)w * (TM(-2) * gam) + (TM)dwr) * q;
                     }
-                    r0c[nn * NT] = (TM)(w * B);
-                    r1c[nn * NT] = (TM)(w * Bh + dwr * B);
                 }
-            } else {
-                for (int k = 0; k < n; ++k) {
-                    const TM gam = (TM)b.gto_gamma[l * n + k];
-                    const TM q = (TM)b.gto_pref[l * n + k] * t_exp2<TM>(-gam * (TM)r2);
-                    r0c[k * NT] = (TM)w * q;
-                    r1c[k * NT] = ((TM)w * (TM(-2) * gam) + (TM)dwr) * q;
+            };
+            // the two radial contractions of lm with M (which carries K_lm)
+            auto hf = [&](int lm, TM& h0, TM& h1) {
+                const TM* mv = Mrow + lm * n;
+                h0 = TM(0);
+                h1 = TM(0);
+                if constexpr (FIXN) {
+                    TM mk[NR];
+                    load_row<TM, NR>(mv, mk);
+    #pragma unroll
+                    for (int k = 0; k < NR; ++k) {
+                        h0 += rr0[k] * mk[k];
+                        h1 += rr1[k] * mk[k];
+                    }
+                } else {
+                    for (int k = 0; k < n; ++k) {
+                        const TM m = mv[k];
+                        h0 += r0c[k * NT] * m;
+                        h1 += r1c[k * NT] * m;
+                    }
                 }
-            }
-        };
-        // the two radial contractions of lm with M (which carries K_lm)
-        auto hf = [&](int lm, TM& h0, TM& h1) {
-            const TM* mv = Mrow + lm * n;
-            h0 = TM(0);
-            h1 = TM(0);
-            if constexpr (FIXN) {
-                TM mk[NR];
-                load_row<TM, NR>(mv, mk);
+            };
+            TM P = TM(0), X = TM(0), Y = TM(0), Z = TM(0);
+            solid_harmonics_adjoint<LMAX, TM>(trx, tryy, trz, on_l, hf, P, X, Y, Z);
+            gx = trx * P + X;
+            gy = tryy * P + Y;
+            gz = trz * P + Z;
+            atomicAdd(&grad[3 * (size_t)a], (double)gx);
+            atomicAdd(&grad[3 * (size_t)a + 1], (double)gy);
+            atomicAdd(&grad[3 * (size_t)a + 2], (double)gz);
+        }
+        if (center_attach && __builtin_amdgcn_readfirstlane(ci) >= 0) {
+            // the wavefront's edges are contiguous, so they span few centers: one masked
+            // wavefront sum per center present, then one LDS add per wavefront and center
+            for (int cc = __builtin_amdgcn_readfirstlane(ci); __any(ci >= cc); ++cc) {
+                double v[3] = {ci == cc ? (double)gx : 0.0, ci == cc ? (double)gy : 0.0, ci == cc ? (double)gz : 0.0};
 #pragma unroll
-                for (int k = 0; k < NR; ++k) {
-                    h0 += rr0[k] * mk[k];
-                    h1 += rr1[k] * mk[k];
-                }
-            } else {
-                for (int k = 0; k < n; ++k) {
-                    const TM m = mv[k];
-                    h0 += r0c[k * NT] * m;
-                    h1 += r1c[k * NT] * m;
-                }
+                for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) v[d] += __shfl_xor(v[d], o, 64);
+                if (lane == 0)
+                    for (int d = 0; d < 3; ++d) atomicAdd(&red[cc][d], v[d]);
             }
-        };
-        TM P = TM(0), X = TM(0), Y = TM(0), Z = TM(0);
-        solid_harmonics_adjoint<LMAX, TM>(trx, tryy, trz, on_l, hf, P, X, Y, Z);
-        gx = trx * P + X;
-        gy = tryy * P + Y;
-        gz = trz * P + Z;
-        atomicAdd(&grad[3 * (size_t)a], (double)gx);
-        atomicAdd(&grad[3 * (size_t)a + 1], (double)gy);
-        atomicAdd(&grad[3 * (size_t)a + 2], (double)gz);
-        sx += (double)gx;
-        sy += (double)gy;
-        sz += (double)gz;
+        }
     }
     if (center_attach) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            sx += __shfl_xor(sx, o, 64);
-            sy += __shfl_xor(sy, o, 64);
-            sz += __shfl_xor(sz, o, 64);
-        }
-        const int lane = tid & 63, wv = tid >> 6;
-        if (lane == 0) {
-            red[0][wv] = sx;
-            red[1][wv] = sy;
-            red[2][wv] = sz;
-        }
         __syncthreads();
-        const int ca = center_attach[c];
-        if (tid == 0 && ca >= 0) {
-            double t0 = 0, t1 = 0, t2 = 0;
-            for (int w2 = 0; w2 < 4; ++w2) {
-                t0 += red[0][w2];
-                t1 += red[1][w2];
-                t2 += red[2][w2];
-            }
-            atomicAdd(&grad[3 * (size_t)ca], -t0);
-            atomicAdd(&grad[3 * (size_t)ca + 1], -t1);
-            atomicAdd(&grad[3 * (size_t)ca + 2], -t2);
+        if (tid < ng) {
+            const int ca = center_attach[c0 + tid];
+            if (ca >= 0)
+                for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)ca + d], -red[tid][d]);
         }
     }
 }
@@ -602,16 +642,20 @@ int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, cons
                    const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
                    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
                    double* grad, hipStream_t st) {
-    const size_t cols = NN > 0 ? 0 : 2 * 256 * (size_t)b.n_max;
-    const size_t smem = sizeof(TM) * ((size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1) + cols) +
+    const size_t cols = sizeof(TM) * (NN > 0 ? 0 : 2 * 256 * (size_t)b.n_max) +
                         (RAD == kRadMixed && NN == 0 ? sizeof(double) * 256 * (size_t)b.n_max : 0);
+    const size_t per_bytes = sizeof(TM) * (size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1);
+    // as many centers per workgroup as fit in 64 KB of coefficient rows (at least one)
+    int ngrp = (int)std::min<size_t>(vjp_group<NN>(), std::max<size_t>(1, (64 * 1024 - std::min<size_t>(cols, 64 * 1024)) / per_bytes));
+    const size_t smem = cols + ngrp * per_bytes;
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: coefficient block too large for LDS");
     auto kern = vjp_gto_kernel<LMAX, TM, RAD, NN>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_edges", st);
-    kern<<<n_centers, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, center_species,
-                                       offsets, nbr, (const TM*)M, grad);
+    kern<<<(n_centers + ngrp - 1) / ngrp, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach,
+                                                           center_species, offsets, nbr, (const TM*)M, n_centers,
+                                                           ngrp, grad);
     prof_end("vjp_edges", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
