@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
                                                    const int32_t* __restrict__ binned_ext,
                                                    const int64_t* __restrict__ offsets,
                                                    int32_t* __restrict__ nbr) {
-    __shared__ int32_t keys[CAP];
+    __shared__ __align__(16) int32_t keys[CAP + 4];
     __shared__ int nkeys;
     const int c = blockIdx.x;
     if (c >= n_centers) return;
@@ -393,29 +393,23 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
         }
     }
     __syncthreads();
+    // ascending order (the reference's nonzero order) by rank: the keys of one center
+    // are distinct extended indices, so a key's position is the number of smaller keys;
+    // four keys per 16-byte LDS read (the same address in every lane), no barrier
     const int n = nkeys < CAP ? nkeys : CAP;
-    int p2 = 1;
-    while (p2 < n) p2 <<= 1;
-    for (int i = n + threadIdx.x; i < p2; i += blockDim.x) keys[i] = INT_MAX;
+    const int n4 = (n + 3) & ~3;
+    for (int i = n + threadIdx.x; i < n4; i += blockDim.x) keys[i] = INT_MAX;
     __syncthreads();
-    for (int k = 2; k <= p2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < p2; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const int a = keys[i], b = keys[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) {
-                        keys[i] = b;
-                        keys[ixj] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
     const int64_t base = offsets[c];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) nbr[base + i] = keys[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int k = keys[i];
+        int rank = 0;
+        for (int j = 0; j < n4; j += 4) {
+            const int4 q = *(const int4*)(keys + j);
+            rank += (q.x < k) + (q.y < k) + (q.z < k) + (q.w < k);
+        }
+        nbr[base + rank] = k;
+    }
 }
 
 }  // namespace
