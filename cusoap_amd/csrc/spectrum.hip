@@ -38,30 +38,58 @@ __device__ __forceinline__ void unpack_feature(uint32_t v, int& j, int& jd, int&
     l = (int)(v >> 28);
 }
 
+// Vector power spectrum: a grid-stride loop over centers.  The feature table and
+// the l prefactors are staged in LDS once per workgroup, and the next center's
+// coefficients are loaded into registers while the current center's features
+// are formed (thread per feature, m summed in order), so the loop is not one
+// global-load latency per center.
+constexpr int kPsPre = 8;        // prefetched coefficients per thread (rows up to 2048 values)
 template <typename TC, typename TO>
-__global__ __launch_bounds__(256) void power_spectrum_kernel(int S, int nmax, int lmax,
+__global__ __launch_bounds__(256) void power_spectrum_kernel(int S, int nmax, int lmax, int n_centers,
                                                              const TC* __restrict__ coeffs,
                                                              const uint32_t* __restrict__ ft, int n_feat,
                                                              TO* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int L2 = (lmax + 1) * (lmax + 1);
     const int per = S * nmax * L2;
-    TC* cs = (TC*)smem;
-    double* pref = (double*)(cs + per + (per & 1));
-    const int c = blockIdx.x;
-    const TC* src = coeffs + (size_t)c * per;
-    for (int i = threadIdx.x; i < per; i += blockDim.x) cs[i] = src[i];
-    if ((int)threadIdx.x <= lmax) pref[threadIdx.x] = sqrt(8.0 * M_PI * M_PI / (2 * threadIdx.x + 1));
-    __syncthreads();
-    TO* dst = out + (size_t)c * n_feat;
-    for (int f = threadIdx.x; f < n_feat; f += blockDim.x) {
-        int j, jd, n, nd, l;
-        unpack_feature(ft[f], j, jd, n, nd, l);
-        const TC* a = cs + (j * nmax + n) * L2 + l * l;
-        const TC* b = cs + (jd * nmax + nd) * L2 + l * l;
-        TC acc = TC(0);
-        for (int m = 0; m < 2 * l + 1; ++m) acc += a[m] * b[m];
-        dst[f] = (TO)((TC)pref[l] * acc);
+    double* pref = (double*)smem;                               // [lmax + 1]
+    uint32_t* fts = (uint32_t*)(pref + 16);                     // [n_feat]
+    TC* cs = (TC*)(fts + ((n_feat + 3) & ~3));                  // [per]
+    const int tid = threadIdx.x;
+    for (int f = tid; f < n_feat; f += 256) fts[f] = ft[f];
+    if (tid <= lmax) pref[tid] = sqrt(8.0 * M_PI * M_PI / (2 * tid + 1));
+    TC pre[kPsPre];
+    auto fetch = [&](int c) {
+        if (c >= n_centers) return;
+        const TC* src = coeffs + (size_t)c * per;
+#pragma unroll
+        for (int j = 0; j < kPsPre; ++j) {
+            const int i = tid + 256 * j;
+            if (i < per) pre[j] = src[i];
+        }
+    };
+    fetch(blockIdx.x);
+    for (int c = blockIdx.x; c < n_centers; c += gridDim.x) {
+#pragma unroll
+        for (int j = 0; j < kPsPre; ++j) {
+            const int i = tid + 256 * j;
+            if (i < per) cs[i] = pre[j];
+        }
+        const TC* src = coeffs + (size_t)c * per;
+        for (int i = 256 * kPsPre + tid; i < per; i += 256) cs[i] = src[i];
+        __syncthreads();
+        fetch(c + gridDim.x);          // in flight while this center's features are formed
+        TO* dst = out + (size_t)c * n_feat;
+        for (int f = tid; f < n_feat; f += 256) {
+            int j, jd, n, nd, l;
+            unpack_feature(fts[f], j, jd, n, nd, l);
+            const TC* a = cs + (j * nmax + n) * L2 + l * l;
+            const TC* b = cs + (jd * nmax + nd) * L2 + l * l;
+            TC acc = TC(0);
+            for (int m = 0; m < 2 * l + 1; ++m) acc += a[m] * b[m];
+            dst[f] = (TO)((TC)pref[l] * acc);
+        }
+        __syncthreads();
     }
 }
 
@@ -214,13 +242,14 @@ int launch_ps(const cusoap_basis_t& b, const void* coeffs, int n_centers, const 
               hipStream_t st) {
     const int L2 = (b.l_max + 1) * (b.l_max + 1);
     const int per = b.n_species * b.n_max * L2;
-    const size_t smem = sizeof(TC) * (size_t)(per + 1) + sizeof(double) * (b.l_max + 2) + 16;
+    const size_t smem = sizeof(double) * 16 + sizeof(uint32_t) * (size_t)((n_feat + 3) & ~3) + sizeof(TC) * (size_t)per;
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "power_spectrum: too many coefficients per center");
     auto kern = power_spectrum_kernel<TC, TO>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    const int grid = n_centers < 2048 ? n_centers : 2048;
     prof_begin("power_spectrum", st);
-    kern<<<n_centers, 256, smem, st>>>(b.n_species, b.n_max, b.l_max, (const TC*)coeffs, ft, n_feat, (TO*)out);
+    kern<<<grid, 256, smem, st>>>(b.n_species, b.n_max, b.l_max, n_centers, (const TC*)coeffs, ft, n_feat, (TO*)out);
     prof_end("power_spectrum", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
