@@ -254,10 +254,10 @@ constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
 constexpr int kVjpGroup = 8;
 template <int NN>
 constexpr int vjp_group() { return NN > 0 ? 1 : kVjpGroup; }
-// the register-table form also gains from 4 wavefronts per SIMD (a few spilled
-// registers cost less than the latency the extra wavefronts hide)
+// the register-table form, which prefetches the next degree's table rows, was
+// measured fastest at 3 wavefronts per SIMD (2.35 ms against 2.47 at 4 and 2.36 unbounded)
 template <int NN>
-constexpr int vjp_min_waves() { return NN > 0 ? 4 : 1; }
+constexpr int vjp_min_waves() { return NN > 0 ? 3 : 1; }
 
 // M row of one lm into registers (16-byte reads when the row allows)
 template <typename T, int N>
@@ -289,6 +289,7 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     constexpr bool MIXED = RAD == kRadMixed;
     constexpr bool FIXN = NN > 0;
     constexpr int NR = FIXN ? NN : 1;
+    constexpr int NQ = FIXN && RAD == kRadTable ? (NN + 1) / 2 : 1;
     const int n = FIXN ? NN : b.n_max, S = b.n_species;
     extern __shared__ __align__(16) unsigned char smem[];
     double* qd = (double*)smem;                             // [n][NT]   (MIXED, runtime n) primitives
@@ -395,15 +396,38 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                 v11 *= tw;
             }
             TM rr0[NR], rr1[NR];                                 // radial vectors of degree l (FIXN)
+            // register table form: node rows of the next degree, (value, slope) pairs of two n per float4
+            float4 nx0[NQ], nx1[NQ];
+            auto load_rows = [&](int l) {
+                if constexpr (FIXN && RAD == kRadTable) {
+                    const float4* a0 = (const float4*)(tb + l * 2 * NN);
+                    const float4* a1 = (const float4*)(tb + (L1 + l) * 2 * NN);
+#pragma unroll
+                    for (int k = 0; k < NQ; ++k) {
+                        nx0[k] = a0[k];
+                        nx1[k] = a1[k];
+                    }
+                }
+            };
+            load_rows(0);
             auto on_l = [&](int l) {
                 if constexpr (FIXN) {
                     const TM tw = (TM)w, tdwr = (TM)dwr;
                     if (RAD == kRadTable) {
-                        const float* p0 = tb + l * 2 * NN;
-                        const float* p1 = p0 + L1 * 2 * NN;
-    #pragma unroll
+                        // this degree's rows were loaded one degree ahead; issue the next degree's
+                        // loads before using them, so their latency overlaps this degree's work
+                        float4 c0[NQ], c1[NQ];
+#pragma unroll
+                        for (int k = 0; k < NQ; ++k) {
+                            c0[k] = nx0[k];
+                            c1[k] = nx1[k];
+                        }
+                        if (l + 1 < L1) load_rows(l + 1);
+#pragma unroll
                         for (int k = 0; k < NN; ++k) {
-                            const TM b0 = (TM)p0[2 * k], d0 = (TM)p0[2 * k + 1], b1 = (TM)p1[2 * k], d1 = (TM)p1[2 * k + 1];
+                            const float4 q0 = c0[k >> 1], q1 = c1[k >> 1];
+                            const TM b0 = (TM)((k & 1) ? q0.z : q0.x), d0 = (TM)((k & 1) ? q0.w : q0.y);
+                            const TM b1 = (TM)((k & 1) ? q1.z : q1.x), d1 = (TM)((k & 1) ? q1.w : q1.y);
                             rr0[k] = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
                             rr1[k] = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
                         }
