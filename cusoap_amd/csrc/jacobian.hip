@@ -19,7 +19,8 @@
 // Three kernels:
 //  rowkey_kernel   one workgroup per center: the center's included edges as
 //                  (output row, edge) keys, bitonic-sorted in LDS -> global.
-//  records_kernel  one workgroup per center, tiles of up to 64 edges: solid
+//  records_kernel  workgroup (center, y) takes the center's edge tiles y, y + gy, ...
+//                  (up to 64 edges each; gy from the mean tiles per center): solid
 //                  harmonics with gradients (thread per edge), radial factors
 //                  (thread per (edge, k, l)), then Bw/Bt and the U/V
 //                  contractions (thread per output) into one float64 record
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
     if (mode == CUSOAP_MODE_PS)
         for (int i = tid; i < S * n * L2; i += nth) Cs[i] = coeffs[(size_t)c * S * n * L2 + i];
     const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
-    for (int64_t t0 = e0; t0 < e1; t0 += TE) {
+    for (int64_t t0 = e0 + (int64_t)blockIdx.y * TE; t0 < e1; t0 += (int64_t)gridDim.y * TE) {
         const int nt = (int)((e1 - t0) < TE ? (e1 - t0) : TE);
         // ---- geometry
         if (tid < nt) {
@@ -198,25 +199,28 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
             rp[0] = rx; rp[1] = ry; rp[2] = rz;
         }
         __syncthreads();
-        // ---- solid harmonics (one item per edge) and radial factors
-        const int n_rad = (RBF == CUSOAP_RBF_GTO) ? nt * NL : nt * n;
-        for (int item = tid; item < nt + n_rad; item += nth) {
-            if (item < nt) {
-                const double* g = geo + item * 8;
-                double* y = Ys + item * 4 * L2;
-                solid_harmonics<LMAX, true, double>(g[0], g[1], g[2], [&](int lm, double v, double gx, double gy, double gz) {
-                    y[lm] = v;
-                    y[L2 + lm] = gx;
-                    y[2 * L2 + lm] = gy;
-                    y[3 * L2 + lm] = gz;
-                });
-            } else if (RBF == CUSOAP_RBF_GTO) {
-                const int it = item - nt;
-                const int e = it / NL, t = it - e * NL;
-                const int k = t / L1, l = t - k * L1;
-                Qs[e * NL + t] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * geo[e * 8 + 3]);
-            } else {
-                const int it = item - nt;
+        // ---- solid harmonics (thread per edge) and radial factors.  The GTO primitives
+        // use a fixed (k, l) slot per thread, decoded once, looping over the tile's edges.
+        if (tid < nt) {
+            const double* g = geo + tid * 8;
+            double* y = Ys + tid * 4 * L2;
+            solid_harmonics<LMAX, true, double>(g[0], g[1], g[2], [&](int lm, double v, double gx, double gy, double gz) {
+                y[lm] = v;
+                y[L2 + lm] = gx;
+                y[2 * L2 + lm] = gy;
+                y[3 * L2 + lm] = gz;
+            });
+        }
+        if (RBF == CUSOAP_RBF_GTO) {
+            const int per_slot = nth / NL;      // threads per (k, l) slot (NL <= 160 < nth)
+            const int slot = tid % NL, e_first = tid / NL;
+            if (e_first < per_slot) {
+                const int k = slot / L1, l = slot - k * L1;
+                const double pref = b.gto_pref[l * n + k], gam = b.gto_gamma[l * n + k];
+                for (int e = e_first; e < nt; e += per_slot) Qs[e * NL + slot] = pref * exp(-gam * geo[e * 8 + 3]);
+            }
+        } else {
+            for (int it = tid; it < nt * n; it += nth) {
                 const int e = it / n, k = it - e * n;
                 const double r = geo[e * 8 + 4];
                 double* qb = Qs + e * NL + k * L1;
@@ -248,46 +252,56 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
             }
         }
         __syncthreads();
-        // ---- Bw / Bt and the U, V contractions (or Y, dY for cc)
-        const int per_uv = (mode == CUSOAP_MODE_PS) ? 4 * R.SNL : 4 * L2;
-        for (int item = tid; item < nt * (NL + per_uv); item += nth) {
-            if (item < nt * NL) {
-                const int e = item / NL, t = item - e * NL;
+        // ---- Bw / Bt: a fixed (n, l) slot per thread, decoded once, looping over the edges
+        {
+            const int per_slot = nth / NL;
+            const int t = tid % NL, e_first = tid / NL;
+            if (e_first < per_slot) {
                 const int nn = t / L1, l = t - nn * L1;
-                double B, Bh;
-                if (RBF == CUSOAP_RBF_GTO) {
-                    const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
-                    B = 0.0;
-                    Bh = 0.0;
-                    for (int k = 0; k < n; ++k) {
-                        const double q = Qs[e * NL + k * L1 + l];
-                        B += beta[k] * q;
-                        Bh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q;
+                const double* beta = (RBF == CUSOAP_RBF_GTO) ? b.gto_beta + ((size_t)l * n + nn) * n : nullptr;
+                for (int e = e_first; e < nt; e += per_slot) {
+                    double B, Bh;
+                    if (RBF == CUSOAP_RBF_GTO) {
+                        B = 0.0;
+                        Bh = 0.0;
+                        for (int k = 0; k < n; ++k) {
+                            const double q = Qs[e * NL + k * L1 + l];
+                            B += beta[k] * q;
+                            Bh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q;
+                        }
+                    } else {
+                        B = Qs[e * NL + t];
+                        Bh = Qh[e * NL + t];
                     }
-                } else {
-                    B = Qs[e * NL + t];
-                    Bh = Qh[e * NL + t];
+                    const double* g = geo + e * 8;
+                    double* rp = rec + (size_t)(t0 + e) * R.stride;
+                    rp[R.offB + t] = g[5] * B;
+                    rp[R.offBt + t] = g[5] * Bh + g[6] * B;
                 }
-                const double* g = geo + e * 8;
-                double* rp = rec + (size_t)(t0 + e) * R.stride;
-                rp[R.offB + t] = g[5] * B;
-                rp[R.offBt + t] = g[5] * Bh + g[6] * B;
-            } else {
-                const int it = item - nt * NL;
-                const int e = it / per_uv, t = it - e * per_uv;
-                double* rp = rec + (size_t)(t0 + e) * R.stride;
-                const double* y = Ys + e * 4 * L2;
-                if (mode == CUSOAP_MODE_PS) {
-                    const int comp = t / R.SNL, rem = t - comp * R.SNL;
-                    const int kq = rem / L1, l = rem - kq * L1;
-                    const double* yv = y + comp * L2 + l * l;
-                    const double* cv = Cs + kq * L2 + l * l;
+            }
+        }
+        // ---- U, V contractions (or Y, dY for cc): thread per output slot t, decoded once
+        // and looping over the tile's edges, with the coefficient row in registers
+        const int per_uv = (mode == CUSOAP_MODE_PS) ? 4 * R.SNL : 4 * L2;
+        for (int t = tid; t < per_uv; t += nth) {
+            double* rp = rec + (size_t)t0 * R.stride + R.offU + t;
+            if (mode == CUSOAP_MODE_PS) {
+                const int comp = t / R.SNL, rem = t - comp * R.SNL;
+                const int kq = rem / L1, l = rem - kq * L1;
+                const int nm = 2 * l + 1;
+                double cr[2 * LMAX + 1];
+#pragma unroll
+                for (int m = 0; m < 2 * LMAX + 1; ++m) cr[m] = m < nm ? Cs[kq * L2 + l * l + m] : 0.0;
+                const double* yv = Ys + comp * L2 + l * l;
+                for (int e = 0; e < nt; ++e) {
                     double acc = 0.0;
-                    for (int m = 0; m < 2 * l + 1; ++m) acc += yv[m] * cv[m];
-                    rp[R.offU + t] = acc;
-                } else {
-                    rp[R.offU + t] = y[t];
+#pragma unroll
+                    for (int m = 0; m < 2 * LMAX + 1; ++m)
+                        if (m < nm) acc += yv[e * 4 * L2 + m] * cr[m];
+                    rp[(size_t)e * R.stride] = acc;
                 }
+            } else {
+                for (int e = 0; e < nt; ++e) rp[(size_t)e * R.stride] = Ys[e * 4 * L2 + t];
             }
         }
         __syncthreads();
@@ -587,22 +601,26 @@ int records_tile(const RecLayout& R, int S, int n, size_t* smem) {
 template <int LMAX>
 int launch_records(const cusoap_basis_t& b, int mode, const double* pos, int n_atoms, const double* shifts,
                    const double* centers, int n_centers, const int64_t* offsets, const int32_t* nbr,
-                   const double* coeffs, double* rec, hipStream_t st) {
+                   int64_t n_edges, const double* coeffs, double* rec, hipStream_t st) {
     const RecLayout R = rec_layout(b.n_species, b.n_max, LMAX, mode);
     size_t smem = 0;
     const int TE = records_tile(R, b.n_species, b.n_max, &smem);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: coefficient block too large for LDS");
+    // several workgroups per center (one per tile of a mean-sized center): one workgroup
+    // per center left most of the GPU idle on a few thousand centers
+    const int64_t mean_tiles = n_centers > 0 ? (n_edges / n_centers + TE - 1) / TE : 1;
+    const dim3 grid((unsigned)n_centers, (unsigned)std::max<int64_t>(1, std::min<int64_t>(16, mean_tiles)));
     prof_begin("jacobian_edges", st);
     if (b.rbf == CUSOAP_RBF_GTO) {
         auto kern = records_kernel<LMAX, CUSOAP_RBF_GTO>;
         if (smem > 64 * 1024)
             CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        kern<<<n_centers, 256, smem, st>>>(b, mode, TE, pos, n_atoms, shifts, centers, offsets, nbr, coeffs, rec);
+        kern<<<grid, 256, smem, st>>>(b, mode, TE, pos, n_atoms, shifts, centers, offsets, nbr, coeffs, rec);
     } else {
         auto kern = records_kernel<LMAX, CUSOAP_RBF_POLY>;
         if (smem > 64 * 1024)
             CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        kern<<<n_centers, 256, smem, st>>>(b, mode, TE, pos, n_atoms, shifts, centers, offsets, nbr, coeffs, rec);
+        kern<<<grid, 256, smem, st>>>(b, mode, TE, pos, n_atoms, shifts, centers, offsets, nbr, coeffs, rec);
     }
     prof_end("jacobian_edges", st);
     CUSOAP_LAUNCH_CHECK();
@@ -689,7 +707,7 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
     CUSOAP_LAUNCH_CHECK();
     int rc = CUSOAP_OK;
 #define CUSOAP_CASE(L) \
-    case L: rc = launch_records<L>(*b, mode, pos, n_atoms, shifts, centers, n_centers, offsets, nbr, coeffs, w.rec, st); break;
+    case L: rc = launch_records<L>(*b, mode, pos, n_atoms, shifts, centers, n_centers, offsets, nbr, n_edges, coeffs, w.rec, st); break;
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
