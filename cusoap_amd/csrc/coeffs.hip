@@ -119,7 +119,10 @@ struct V2<double> {
 template <int PREC>
 constexpr int coeff_min_waves() { return PREC == CUSOAP_PREC_F64 ? 3 : 4; }
 
-template <int LMAX, int PREC, int RBF>
+// SB: species accumulated per pass over the edges (2 when the system has at most 2,
+// else 4): the accumulators of unused species would otherwise hold registers and
+// push the kernel into spills at 4 wavefronts per SIMD
+template <int LMAX, int PREC, int RBF, int SB>
 __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kernel(cusoap_basis_t b, const double* __restrict__ pos, int n_atoms,
                                                          const int32_t* __restrict__ atom_sp,
                                                          const double* __restrict__ shifts,
@@ -171,8 +174,8 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const TA tab_h = RBF == kRbfTable ? (TA)(b.rad_rmax / b.rad_intervals) : TA(0);
     __syncthreads();
 
-    for (int s0 = 0; s0 < S; s0 += 4) {
-        const int s1 = (s0 + 4 < S) ? s0 + 4 : S;
+    for (int s0 = 0; s0 < S; s0 += SB) {
+        const int s1 = (s0 + SB < S) ? s0 + SB : S;
         for (int q0 = 0; q0 < nslot_all; q0 += 64) {
             // ---- this lane's slot: n pair p (n0 = 2p, n1 = 2p+1) x lm group g of degree l
             const int qs = q0 + lane;
@@ -194,9 +197,9 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 cnt = left < 4 ? left : 4;
             }
             const int roff = l * nP + n0, yoff = 4 * grp;
-            TA acc[4][2][4];
+            TA acc[SB][2][4];
 #pragma unroll
-            for (int a4 = 0; a4 < 4; ++a4)
+            for (int a4 = 0; a4 < SB; ++a4)
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -226,9 +229,9 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 // species + rank of the edge among that species' edges (ballot + mbcnt), the
                 // same in every wavefront; LDS rows are indexed by p, so the loops below run
                 // over contiguous uniform ranges instead of walking lane masks
-                int beg[4], scnt[4], pidx = -1, nv = 0;
+                int beg[SB], scnt[SB], pidx = -1, nv = 0;
 #pragma unroll
-                for (int a4 = 0; a4 < 4; ++a4) {
+                for (int a4 = 0; a4 < SB; ++a4) {
                     const uint64_t m = __ballot(sp == s0 + a4);
                     const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                     beg[a4] = nv;
@@ -355,7 +358,7 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 // ---- B: wavefront q sums the positions p = q (mod 4) of each species range
                 if (valid) {
 #pragma unroll
-                    for (int a4 = 0; a4 < 4; ++a4) {
+                    for (int a4 = 0; a4 < SB; ++a4) {
                         if (s0 + a4 >= s1) continue;
                         auto fma_edge = [&](int e) {
                             const typename V2<TA>::type rv = *(const typename V2<TA>::type*)(Rs + e * RS + roff);
@@ -381,11 +384,11 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 __syncthreads();
             }
             // ---- combine the four wavefronts' partial sums in a fixed order (LDS, aliases the tile)
-            TA* part = Rs;                                  // [4 species][n][L2]
+            TA* part = Rs;                                  // [SB species][n][L2]
             for (int w2 = 0; w2 < 4; ++w2) {
                 if (wv == w2 && valid) {
 #pragma unroll
-                    for (int a4 = 0; a4 < 4; ++a4) {
+                    for (int a4 = 0; a4 < SB; ++a4) {
                         if (s0 + a4 >= s1) continue;
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
@@ -455,7 +458,7 @@ int launch_coeff(const cusoap_basis_t& b, const double* pos, int n_atoms, const 
     const size_t tab = RBF == kRbfTable ? (sizeof(int32_t) + 4 * sizeof(TA)) * (size_t)kTE : 0;
     const size_t smem = gcon + tab + ys_bytes + (part > rs_bytes ? part : rs_bytes);
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: LDS tile too large");
-    auto kern = coeff_kernel<LMAX, PREC, RBF>;
+    auto kern = b.n_species <= 2 ? coeff_kernel<LMAX, PREC, RBF, 2> : coeff_kernel<LMAX, PREC, RBF, 4>;
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("coefficients", st);
