@@ -127,15 +127,25 @@ __global__ void bbox_partial_kernel(const double* __restrict__ c, int n, double*
         for (int d = 0; d < 6; ++d) part[6 * blockIdx.x + d] = red[d][0];
 }
 
+// one wavefront: lane-strided partial boxes, then a shuffle reduction (min and max are
+// exact, so the result does not depend on the order)
 __global__ void bbox_final_kernel(double* part, int nblk) {
-    if (threadIdx.x != 0) return;
+    const int lane = threadIdx.x;
     double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nblk; ++b)
+    for (int b = lane; b < nblk; b += 64)
         for (int d = 0; d < 3; ++d) {
             r[d] = fmin(r[d], part[6 * b + d]);
             r[3 + d] = fmax(r[3 + d], part[6 * b + 3 + d]);
         }
-    for (int d = 0; d < 6; ++d) part[d] = r[d];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            r[d] = fmin(r[d], __shfl_xor(r[d], o, 64));
+            r[3 + d] = fmax(r[3 + d], __shfl_xor(r[3 + d], o, 64));
+        }
+    __syncthreads();                  // every lane has read its partials before lane 0 overwrites them
+    if (lane == 0)
+        for (int d = 0; d < 6; ++d) part[d] = r[d];
 }
 
 // --- exclusive scan int32 -> int64 ---------------------------------------------

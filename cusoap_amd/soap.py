@@ -266,6 +266,7 @@ class SOAP:
         self._build_feature_tables()
         self._basis_struct = None
         self._basis_learned = None
+        self._species_cache = None   # device tensors derived from (Z, cell, pbc) of the last system
         self._last_nl_backend = "hip-cell-list"
         if radial not in ("exact", "table", "learned"):
             raise ValueError("radial must be 'exact', 'table' or 'learned'")
@@ -545,6 +546,10 @@ class SOAP:
     def _context(self, system, centers) -> _Ctx:
         self._require_gpu()
         pos, Z, cell, pbc = system_to_arrays(system, keep_device=True)
+        if centers is None and self._species_cache is not None:
+            x = self._context_cached(pos, Z, cell, pbc)
+            if x is not None:
+                return x
         bad = set(np.unique(Z).tolist()) - self._atomic_number_set
         if bad:
             raise ValueError(f"System contains atomic numbers not in species list: {sorted(bad)}")
@@ -578,6 +583,31 @@ class SOAP:
         x.shifts = self._dev(shifts)
         self_sp = np.where(cidx >= 0, sp[np.maximum(cidx, 0)] if sp.size else -1, -1).astype(np.int32)
         x.self_sp = self._dev(self_sp)
+        if centers is None:
+            # the same composition, cell and pbc on the next call (an MD or benchmark loop)
+            # reuse these without the host work and the synchronous host-to-device copies
+            self._species_cache = (np.array(Z, copy=True),
+                                   None if cell is None else np.array(cell, dtype=np.float64, copy=True),
+                                   None if pbc is None else np.array(pbc, dtype=bool, copy=True), x)
+        return x
+
+    def _context_cached(self, pos, Z, cell, pbc):
+        """Context of a system with the cached composition, cell and pbc, else None."""
+        Zc, cellc, pbcc, xc = self._species_cache
+        if Z.shape != Zc.shape or not np.array_equal(Z, Zc):
+            return None
+        if self.periodic:
+            if cell is None or pbc is None or cellc is None or pbcc is None:
+                return None
+            if not (np.array_equal(np.asarray(cell, dtype=np.float64), cellc) and
+                    np.array_equal(np.asarray(pbc, dtype=bool), pbcc)):
+                return None
+        x = _Ctx()
+        x.pos = pos.to(self.device) if isinstance(pos, torch.Tensor) else self._dev(pos)
+        x.n_atoms = pos.shape[0]
+        x.centers = x.pos
+        for k in ("n_centers", "n_img", "Z_h", "cidx_h", "sp_h", "sp", "shifts", "self_sp"):
+            setattr(x, k, getattr(xc, k))
         return x
 
     # ---- pipeline stages ----------------------------------------------------------

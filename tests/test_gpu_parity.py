@@ -338,3 +338,25 @@ def test_nl_distance_rounding_at_cutoff():
     off, nbr = g.neighbor_list(s, drop_self=True)
     counts = np.diff(off.cpu().numpy())
     assert counts[0] == (1 if sep_in else 0)
+
+
+def test_context_cache_follows_system_changes():
+    """Reused species/cell/shift tensors (same composition, cell and pbc as the last call)
+    must give what a fresh object gives, and any change in species, cell or pbc must be
+    picked up."""
+    p = dict(species=[1, 8], r_cut=4.0, n_max=4, l_max=3, sigma=0.5, periodic=True)
+    s1 = systems.water_box(4, seed=11)
+    g = SOAP(device="cuda:0", dtype="float64", **p)
+    g.create(s1)
+    d1 = g.create(dict(s1, positions=np.asarray(s1["positions"]) + 0.01))       # cached path
+    ref = SOAP(device="cuda:0", dtype="float64", **p).create(dict(s1, positions=np.asarray(s1["positions"]) + 0.01))
+    assert torch.equal(d1, ref)
+    Z2 = np.asarray(s1["atomic_numbers"]).copy()
+    Z2[[0, 1]] = Z2[[1, 0]]                                                       # same shape, other species
+    s2 = dict(s1, atomic_numbers=Z2)
+    assert torch.equal(g.create(s2), SOAP(device="cuda:0", dtype="float64", **p).create(s2))
+    cell3 = np.asarray(s1["cell"], dtype=np.float64) * 1.05                        # other cell
+    s3 = dict(s2, cell=cell3)
+    assert torch.equal(g.create(s3), SOAP(device="cuda:0", dtype="float64", **p).create(s3))
+    s4 = dict(s3, pbc=[True, True, False])                                          # other pbc
+    assert torch.equal(g.create(s4), SOAP(device="cuda:0", dtype="float64", **p).create(s4))
