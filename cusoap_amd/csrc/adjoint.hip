@@ -252,8 +252,15 @@ constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
 // lanes busy with groups of 8; the register-table form is bound by the latency of
 // its table reads and measured fastest with one center per workgroup
 constexpr int kVjpGroup = 8;
+// register-table form: 4 neighbouring centers per workgroup, their force contributions
+// summed per distinct neighbour atom in an LDS hash table (float sums, linear probing)
+// before one float64 atomic per atom and component: the neighbourhoods of consecutive
+// centers overlap, so this cuts the global atomics (executed at the memory side) ~2x
+constexpr int kVjpHashGroup = 4;
+constexpr int kVjpHashSlots = 1024;
+constexpr int kVjpHashProbe = 32;      // probes before an edge falls back to global atomics
 template <int NN>
-constexpr int vjp_group() { return NN > 0 ? 1 : kVjpGroup; }
+constexpr int vjp_group() { return NN > 0 ? kVjpHashGroup : kVjpGroup; }
 // the register-table form, which prefetches the next degree's table rows, was
 // measured fastest at 3 wavefronts per SIMD (2.35 ms against 2.47 at 4 and 2.36 unbounded)
 template <int NN>
@@ -300,7 +307,15 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     __shared__ double gcen[kVjpGroup][3];
     __shared__ int gslot[kVjpGroup];                        // table slot of each center
     __shared__ double red[kVjpGroup][3];                    // center-attach sums
+    constexpr bool HASH = FIXN && RAD == kRadTable;
+    __shared__ int hkey[HASH ? kVjpHashSlots : 1];          // neighbour atom of each slot or -1
+    __shared__ float hval[HASH ? kVjpHashSlots : 1][3];     // its summed contributions
     const int tid = threadIdx.x, lane = tid & 63;
+    if (HASH)
+        for (int i = tid; i < kVjpHashSlots; i += NT) {
+            hkey[i] = -1;
+            hval[i][0] = hval[i][1] = hval[i][2] = 0.0f;
+        }
     const int c0 = blockIdx.x * ngrp, ng = n_centers - c0 < ngrp ? n_centers - c0 : ngrp;
     const int per = S * n * L2;
     if (tid <= ng) goff[tid] = offsets[c0 + tid];
@@ -523,9 +538,25 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             gx = trx * P + X;
             gy = tryy * P + Y;
             gz = trz * P + Z;
-            atomicAdd(&grad[3 * (size_t)a], (double)gx);
-            atomicAdd(&grad[3 * (size_t)a + 1], (double)gy);
-            atomicAdd(&grad[3 * (size_t)a + 2], (double)gz);
+            bool done = false;
+            if constexpr (HASH) {
+                unsigned h = ((unsigned)a * 2654435761u) >> 22;      // 10-bit multiplicative hash
+                for (int pr = 0; pr < kVjpHashProbe; ++pr, h = (h + 1) & (kVjpHashSlots - 1)) {
+                    const int k = atomicCAS(&hkey[h], -1, a);
+                    if (k == -1 || k == a) {
+                        atomicAdd(&hval[h][0], (float)gx);
+                        atomicAdd(&hval[h][1], (float)gy);
+                        atomicAdd(&hval[h][2], (float)gz);
+                        done = true;
+                        break;
+                    }
+                }
+            }
+            if (!done) {
+                atomicAdd(&grad[3 * (size_t)a], (double)gx);
+                atomicAdd(&grad[3 * (size_t)a + 1], (double)gy);
+                atomicAdd(&grad[3 * (size_t)a + 2], (double)gz);
+            }
         }
         if (center_attach && __builtin_amdgcn_readfirstlane(ci) >= 0) {
             // the wavefront's edges are contiguous, so they span few centers: one masked
@@ -541,13 +572,17 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             }
         }
     }
-    if (center_attach) {
-        __syncthreads();
-        if (tid < ng) {
-            const int ca = center_attach[c0 + tid];
-            if (ca >= 0)
-                for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)ca + d], -red[tid][d]);
+    if (HASH || center_attach) __syncthreads();
+    if (HASH)
+        for (int i = tid; i < kVjpHashSlots; i += NT) {
+            const int a = hkey[i];
+            if (a >= 0)
+                for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)a + d], (double)hval[i][d]);
         }
+    if (center_attach && tid < ng) {
+        const int ca = center_attach[c0 + tid];
+        if (ca >= 0)
+            for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)ca + d], -red[tid][d]);
     }
 }
 
