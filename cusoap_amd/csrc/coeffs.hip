@@ -205,13 +205,18 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
 #pragma unroll
                     for (int j = 0; j < 4; ++j) acc[a4][h][j] = TA(0);
 
+            // neighbour indices are loaded one tile ahead, so the first load of the
+            // nbr -> position chain is off each tile's critical path
+            int ext_next = (e0 + lane < e1) ? nbr[e0 + lane] : 0;
             for (int64_t t0 = e0; t0 < e1; t0 += kTE) {
                 const int nt = (int)((e1 - t0) < kTE ? (e1 - t0) : kTE);
+                const int ext_cur = ext_next;
+                if (t0 + kTE + lane < e1) ext_next = nbr[t0 + kTE + lane];
                 // ---- A: lane = edge; wavefront q takes m = q, l = q (mod 4)
                 int sp = -1;
                 double rx = 0.0, ry = 0.0, rz = 0.0, r2 = 0.0, r = 0.0, w = 1.0;
                 if (lane < nt) {
-                    const int ext = nbr[t0 + lane];
+                    const int ext = ext_cur;
                     const int img = ext / n_atoms, a = ext - img * n_atoms;
                     rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
                     ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
