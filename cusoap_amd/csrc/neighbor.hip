@@ -296,24 +296,21 @@ __device__ __forceinline__ bool edge_test(double cx, double cy, double cz, doubl
 }
 
 // The candidate cells of a center: up to 3 x 3 rows of cells that are
-// consecutive in z, hence contiguous ranges of the binned arrays.  Candidate k
-// (0 <= k < total) lives at binned[start[r] + k - pre[r]] for the last row r
-// with pre[r] <= k.  Every field is uniform across the wavefront.
+// consecutive in z, hence contiguous ranges binned[start[r], start[r] + len[r])
+// of the binned arrays.  Every field is uniform across the wavefront.
 struct CandRows {
     int64_t start[9];
-    int64_t pre[9];
-    int64_t total;
+    int len[9];                 // candidates in each cell row (0 for unused rows)
 };
 
 __device__ __forceinline__ CandRows cand_rows(const Grid& g, const CenterCells& cc,
                                               const int64_t* __restrict__ cell_start) {
     CandRows cr;
     int r = 0;
-    int64_t tot = 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         cr.start[i] = 0;
-        cr.pre[i] = LLONG_MAX;
+        cr.len[i] = 0;
     }
     for (int ix = cc.x0; ix <= cc.x1; ++ix)
         for (int iy = cc.y0; iy <= cc.y1; ++iy) {
@@ -323,24 +320,14 @@ __device__ __forceinline__ CandRows cand_rows(const Grid& g, const CenterCells& 
             for (int i = 0; i < 9; ++i)
                 if (i == r) {
                     cr.start[i] = s0;
-                    cr.pre[i] = tot;
+                    cr.len[i] = (int)(s1 - s0);
                 }
-            tot += s1 - s0;
             ++r;
         }
-    cr.total = tot;
     return cr;
 }
 
-__device__ __forceinline__ int64_t cand_index(const CandRows& cr, int64_t k) {
-    int64_t idx = cr.start[0] + k;
-#pragma unroll
-    for (int i = 1; i < 9; ++i)
-        if (k >= cr.pre[i]) idx = cr.start[i] + (k - cr.pre[i]);
-    return idx;
-}
-
-// one wavefront per center; the lanes stride through the flattened candidates
+// one wavefront per center; the lanes stride through each candidate cell row
 __global__ __launch_bounds__(256) void count_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
                                                     int n_centers, double cutoff2, int drop_self,
                                                     const int64_t* __restrict__ cell_start,
@@ -353,10 +340,13 @@ __global__ __launch_bounds__(256) void count_kernel(cusoap_nl_plan_t plan, const
     const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
     const double cn = sqnorm_rn(cx, cy, cz);
     const CandRows cr = cand_rows(g, center_cells(g, cx, cy, cz), cell_start);
+    // row by row (uniform bounds): no per-candidate search for the row of a flattened index
     int cnt = 0;
-#pragma unroll 4
-    for (int64_t k = lane; k < cr.total; k += 64)
-        cnt += edge_test(cx, cy, cz, cn, binned[cand_index(cr, k)], cutoff2, drop_self) ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const double4* row = binned + cr.start[i];
+        for (int k = lane; k < cr.len[i]; k += 64) cnt += edge_test(cx, cy, cz, cn, row[k], cutoff2, drop_self) ? 1 : 0;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if (lane == 0) counts[c] = cnt;
@@ -394,12 +384,18 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
     const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
     const double cn = sqnorm_rn(cx, cy, cz);
     const CandRows cr = cand_rows(g, center_cells(g, cx, cy, cz), cell_start);
-#pragma unroll 2
-    for (int64_t k = threadIdx.x; k < cr.total; k += blockDim.x) {
-        const int64_t idx = cand_index(cr, k);
-        if (edge_test(cx, cy, cz, cn, binned[idx], cutoff2, drop_self)) {
-            const int slot = atomicAdd(&nkeys, 1);
-            if (slot < CAP) keys[slot] = binned_ext[idx];
+    // wavefront q takes cell rows q, q + 4, q + 8 (uniform bounds, no per-candidate row
+    // search); the keys are put in order below
+    const int wq = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if ((i & 3) != wq) continue;
+        const int64_t s0 = cr.start[i];
+        for (int k = lane; k < cr.len[i]; k += 64) {
+            if (edge_test(cx, cy, cz, cn, binned[s0 + k], cutoff2, drop_self)) {
+                const int slot = atomicAdd(&nkeys, 1);
+                if (slot < CAP) keys[slot] = binned_ext[s0 + k];
+            }
         }
     }
     __syncthreads();
