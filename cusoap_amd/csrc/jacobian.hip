@@ -20,12 +20,13 @@
 //  rowkey_kernel   one workgroup per center: the center's included edges as
 //                  (output row, edge) keys, bitonic-sorted in LDS -> global.
 //  records_kernel  workgroup (center, y) takes the center's edge tiles y, y + gy, ...
-//                  (up to 64 edges each; gy from the mean tiles per center): solid
+//                  (up to 32 edges each; gy from the mean tiles per center): solid
 //                  harmonics with gradients (thread per edge), radial factors
 //                  (thread per (edge, k, l)), then Bw/Bt and the U/V
-//                  contractions (thread per output) into one float64 record
-//                  per edge:  rvec[3] | Bw[n][l] | Bt[n][l] | ps: U[Sn][l], V[3][Sn][l]
-//                                                              cc: Y[L2], dY[3][L2]
+//                  contractions (thread per output) into one record per edge,
+//                  float32 when the rows are computed in float32, else float64:
+//                      rvec[3] | Bw[n][l] | Bt[n][l] | ps: U[Sn][l], V[3][Sn][l]
+//                                                      cc: Y[L2], dY[3][L2]
 //  dense_kernel    one workgroup per (center, 64-row chunk): each run of
 //                  consecutive rows without an edge is stored by the whole
 //                  workgroup as one flat range of 16-byte zeros; then each
@@ -157,14 +158,14 @@ __global__ __launch_bounds__(256) void rowkey_kernel(int n_atoms, const int64_t*
 // ---------------------------------------------------------------------------
 // per-edge records
 // ---------------------------------------------------------------------------
-template <int LMAX, int RBF>
+template <int LMAX, int RBF, typename TR>
 __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode, int TE,
                                                       const double* __restrict__ pos, int n_atoms,
                                                       const double* __restrict__ shifts,
                                                       const double* __restrict__ centers,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr,
-                                                      const double* __restrict__ coeffs, double* __restrict__ rec) {
+                                                      const double* __restrict__ coeffs, TR* __restrict__ rec) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     const int n = b.n_max, S = b.n_species, NL = n * L1;
     const RecLayout R = rec_layout(S, n, LMAX, mode);
@@ -195,8 +196,8 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
             if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
             double* g = geo + tid * 8;
             g[0] = rx; g[1] = ry; g[2] = rz; g[3] = r2; g[4] = r; g[5] = w; g[6] = r > 0.0 ? dw / r : 0.0;
-            double* rp = rec + (size_t)e * R.stride;
-            rp[0] = rx; rp[1] = ry; rp[2] = rz;
+            TR* rp = rec + (size_t)e * R.stride;
+            rp[0] = (TR)rx; rp[1] = (TR)ry; rp[2] = (TR)rz;
         }
         __syncthreads();
         // ---- solid harmonics (thread per edge) and radial factors.  The GTO primitives
@@ -274,9 +275,9 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
                         Bh = Qh[e * NL + t];
                     }
                     const double* g = geo + e * 8;
-                    double* rp = rec + (size_t)(t0 + e) * R.stride;
-                    rp[R.offB + t] = g[5] * B;
-                    rp[R.offBt + t] = g[5] * Bh + g[6] * B;
+                    TR* rp = rec + (size_t)(t0 + e) * R.stride;
+                    rp[R.offB + t] = (TR)(g[5] * B);
+                    rp[R.offBt + t] = (TR)(g[5] * Bh + g[6] * B);
                 }
             }
         }
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
         // and looping over the tile's edges, with the coefficient row in registers
         const int per_uv = (mode == CUSOAP_MODE_PS) ? 4 * R.SNL : 4 * L2;
         for (int t = tid; t < per_uv; t += nth) {
-            double* rp = rec + (size_t)t0 * R.stride + R.offU + t;
+            TR* rp = rec + (size_t)t0 * R.stride + R.offU + t;
             if (mode == CUSOAP_MODE_PS) {
                 const int comp = t / R.SNL, rem = t - comp * R.SNL;
                 const int kq = rem / L1, l = rem - kq * L1;
@@ -298,10 +299,10 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
 #pragma unroll
                     for (int m = 0; m < 2 * LMAX + 1; ++m)
                         if (m < nm) acc += yv[e * 4 * L2 + m] * cr[m];
-                    rp[(size_t)e * R.stride] = acc;
+                    rp[(size_t)e * R.stride] = (TR)acc;
                 }
             } else {
-                for (int e = 0; e < nt; ++e) rp[(size_t)e * R.stride] = Ys[e * 4 * L2 + t];
+                for (int e = 0; e < nt; ++e) rp[(size_t)e * R.stride] = (TR)Ys[e * 4 * L2 + t];
             }
         }
         __syncthreads();
@@ -325,38 +326,41 @@ __device__ __forceinline__ FeatDec dec(uint32_t v) {
     return f;
 }
 
-__device__ __forceinline__ double ps_term(const double* rec, const RecLayout& R, int nmax, int s, int d,
+template <typename TR>
+__device__ __forceinline__ double ps_term(const TR* rec, const RecLayout& R, int nmax, int s, int d,
                                           const FeatDec& f) {
     const double rv = rec[d];
-    const double* B = rec + R.offB;
-    const double* Bt = rec + R.offBt;
-    const double* U = rec + R.offU;
-    const double* V = rec + R.offV + d * R.SNL;
+    const TR* B = rec + R.offB;
+    const TR* Bt = rec + R.offBt;
+    const TR* U = rec + R.offU;
+    const TR* V = rec + R.offV + d * R.SNL;
     double v = 0.0;
     if (f.j == s) {
         const int a = f.n * R.L1 + f.l, q = (f.jd * nmax + f.nd) * R.L1 + f.l;
-        v += rv * Bt[a] * U[q] + B[a] * V[q];
+        v += rv * (double)Bt[a] * (double)U[q] + (double)B[a] * (double)V[q];
     }
     if (f.jd == s) {
         const int a = f.nd * R.L1 + f.l, q = (f.j * nmax + f.n) * R.L1 + f.l;
-        v += rv * Bt[a] * U[q] + B[a] * V[q];
+        v += rv * (double)Bt[a] * (double)U[q] + (double)B[a] * (double)V[q];
     }
     return v;
 }
 
-__device__ __forceinline__ double cc_term(const double* rec, const RecLayout& R, int nmax, int s, int d, int f) {
+template <typename TR>
+__device__ __forceinline__ double cc_term(const TR* rec, const RecLayout& R, int nmax, int s, int d, int f) {
     const int per_s = nmax * R.L2;
     const int fs = f / per_s;
     if (fs != s) return 0.0;
     const int rem = f - fs * per_s;
     const int nn = rem / R.L2, lmp = rem - nn * R.L2;
     const int lm = cc_src(lmp), l = lm_l(lm);
-    const double v = rec[d] * rec[R.offBt + nn * R.L1 + l] * rec[R.offU + lm] +
-                     rec[R.offB + nn * R.L1 + l] * rec[R.offV + d * R.L2 + lm];
+    const double v = (double)rec[d] * (double)rec[R.offBt + nn * R.L1 + l] * (double)rec[R.offU + lm] +
+                     (double)rec[R.offB + nn * R.L1 + l] * (double)rec[R.offV + d * R.L2 + lm];
     return lmp >= 4 ? v * 0.17958712212516656 /* pi^-3/2 */ : v;
 }
 
-__device__ __forceinline__ double elem_value(const double* rp, const RecLayout& R, int mode, int nmax, int s, int t,
+template <typename TR>
+__device__ __forceinline__ double elem_value(const TR* rp, const RecLayout& R, int mode, int nmax, int s, int t,
                                              int n_feat, const uint32_t* fts, const double* prefs) {
     const int d = t / n_feat, f = t - d * n_feat;
     if (mode == CUSOAP_MODE_PS) {
@@ -434,7 +438,7 @@ __device__ __forceinline__ TC ps_elem(const TC* rp, const RecLayout& R, int s, i
 }
 
 // VEC: the row length is a multiple of 4 elements and the output is 16-byte aligned.
-// TC: arithmetic type of the single-edge rows (records are float64 in memory).
+// TC: arithmetic type of the single-edge rows and the element type of the records.
 template <typename TO, typename TC, bool VEC>
 __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, int n_atoms,
                                                     const int32_t* __restrict__ atom_sp,
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, 
                                                     const unsigned long long* __restrict__ rowkeys,
                                                     const int32_t* __restrict__ chunk_lo, int n_inc, int chunks,
                                                     const uint32_t* __restrict__ ft, int n_feat,
-                                                    const double* __restrict__ rec, TO* __restrict__ out) {
+                                                    const TC* __restrict__ rec, TO* __restrict__ out) {
     __shared__ unsigned long long ckeys[kChunkKeys];
     __shared__ int rstart[kRowsPerBlock + 1];
     __shared__ int nzrows[kRowsPerBlock];
@@ -531,8 +535,8 @@ __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, 
         if (q1 - q0 == 1 && r != att_row) {
             // one edge: stage its record in LDS (wavefront-private slot)
             const int64_t e = e0 + (int64_t)(keys[q0] & 0xffffffffull);
-            const double* rp = rec + (size_t)e * R.stride;
-            for (int k = lane; k < R.stride; k += 64) myrec[k] = (TC)rp[k];
+            const TC* rp = rec + (size_t)e * R.stride;
+            for (int k = lane; k < R.stride; k += 64) myrec[k] = rp[k];
             wave_sync();
             const int s = atom_sp[nbr[e] % n_atoms];
             if (mode == CUSOAP_MODE_PS && VEC) {
@@ -563,14 +567,14 @@ __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, 
                     dst[t] = (TO)ps_elem<TC>(myrec, R, s, d, plan[f], prefc);
                 }
             } else {
-                const double* rpd = rec + (size_t)e * R.stride;
+                const TC* rpd = rec + (size_t)e * R.stride;
                 for (int t = lane; t < row_len; t += 64)
                     dst[t] = (TO)elem_value(rpd, R, mode, n, s, t, n_feat, fts, prefs);
             }
             wave_sync();
             continue;
         }
-        // several edges (periodic images) and/or the attach row: float64 from the global records
+        // several edges (periodic images) and/or the attach row: float64 sums of the global records
         for (int t = lane; t < row_len; t += 64) {
             double v = 0.0;
             for (int q = q0; q < q1; ++q) {
@@ -592,16 +596,18 @@ int records_tile(const RecLayout& R, int S, int n, size_t* smem) {
     const size_t per = 4 * (size_t)R.L2 + 2 * (size_t)R.NL + 8;
     size_t room = 8192 > fixed ? 8192 - fixed : 0;     // doubles in 64 KiB
     size_t te = room / per;
-    if (te > 64) te = 64;
+    // at most 32 edges per tile: ~35 KiB of LDS at n 7, l 3 leaves room for 4 workgroups
+    // per CU (64-edge tiles, 2 per CU, were 25% slower)
+    if (te > 32) te = 32;
     if (te < 8) te = 8;
     *smem = sizeof(double) * (fixed + te * per);
     return (int)te;
 }
 
-template <int LMAX>
+template <int LMAX, typename TR>
 int launch_records(const cusoap_basis_t& b, int mode, const double* pos, int n_atoms, const double* shifts,
                    const double* centers, int n_centers, const int64_t* offsets, const int32_t* nbr,
-                   int64_t n_edges, const double* coeffs, double* rec, hipStream_t st) {
+                   int64_t n_edges, const double* coeffs, TR* rec, hipStream_t st) {
     const RecLayout R = rec_layout(b.n_species, b.n_max, LMAX, mode);
     size_t smem = 0;
     const int TE = records_tile(R, b.n_species, b.n_max, &smem);
@@ -612,12 +618,12 @@ int launch_records(const cusoap_basis_t& b, int mode, const double* pos, int n_a
     const dim3 grid((unsigned)n_centers, (unsigned)std::max<int64_t>(1, std::min<int64_t>(16, mean_tiles)));
     prof_begin("jacobian_edges", st);
     if (b.rbf == CUSOAP_RBF_GTO) {
-        auto kern = records_kernel<LMAX, CUSOAP_RBF_GTO>;
+        auto kern = records_kernel<LMAX, CUSOAP_RBF_GTO, TR>;
         if (smem > 64 * 1024)
             CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         kern<<<grid, 256, smem, st>>>(b, mode, TE, pos, n_atoms, shifts, centers, offsets, nbr, coeffs, rec);
     } else {
-        auto kern = records_kernel<LMAX, CUSOAP_RBF_POLY>;
+        auto kern = records_kernel<LMAX, CUSOAP_RBF_POLY, TR>;
         if (smem > 64 * 1024)
             CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         kern<<<grid, 256, smem, st>>>(b, mode, TE, pos, n_atoms, shifts, centers, offsets, nbr, coeffs, rec);
@@ -631,7 +637,7 @@ template <typename TO, typename TC, bool VEC>
 int launch_dense(const cusoap_basis_t& b, int mode, int n_atoms, const int32_t* atom_sp, const int32_t* center_attach,
                  const int64_t* offsets, const int32_t* nbr, const int32_t* inv_atom,
                  const unsigned long long* rowkeys, const int32_t* chunk_lo, int n_centers, int n_inc,
-                 const uint32_t* ft, int n_feat, const double* rec, void* out, hipStream_t st) {
+                 const uint32_t* ft, int n_feat, const TC* rec, void* out, hipStream_t st) {
     const RecLayout R = rec_layout(b.n_species, b.n_max, b.l_max, mode);
     if (R.SNL >= 4096 || R.NL >= 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: too many radial channels");
     const size_t smem = sizeof(double) * 32 + (sizeof(unsigned long long) + sizeof(uint32_t)) * (size_t)n_feat +
@@ -706,8 +712,15 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
                                              w.chunk_lo, w.err);
     CUSOAP_LAUNCH_CHECK();
     int rc = CUSOAP_OK;
+    // records in float32 when the rows are computed in float32 (half the record traffic)
+    const bool f32 = odt == CUSOAP_DT_F32 && cdt == CUSOAP_DT_F32;
 #define CUSOAP_CASE(L) \
-    case L: rc = launch_records<L>(*b, mode, pos, n_atoms, shifts, centers, n_centers, offsets, nbr, n_edges, coeffs, w.rec, st); break;
+    case L: \
+        rc = f32 ? launch_records<L, float>(*b, mode, pos, n_atoms, shifts, centers, n_centers, offsets, nbr, n_edges, \
+                                            coeffs, (float*)w.rec, st) \
+                 : launch_records<L, double>(*b, mode, pos, n_atoms, shifts, centers, n_centers, offsets, nbr, n_edges, \
+                                             coeffs, w.rec, st); \
+        break;
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
@@ -719,8 +732,8 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
     const bool vec = ((3 * (size_t)n_feat) % 4 == 0) && (((uintptr_t)out & 15) == 0);
 #define CUSOAP_DENSE(TO, TC, V) \
     launch_dense<TO, TC, V>(*b, mode, n_atoms, atom_sp, center_attach, offsets, nbr, inv_atom, w.rowkeys, w.chunk_lo, \
-                            n_centers, n_inc, ft, n_feat, w.rec, out, st)
-    if (odt == CUSOAP_DT_F32 && cdt == CUSOAP_DT_F32) rc = vec ? CUSOAP_DENSE(float, float, true) : CUSOAP_DENSE(float, float, false);
+                            n_centers, n_inc, ft, n_feat, (const TC*)w.rec, out, st)
+    if (f32) rc = vec ? CUSOAP_DENSE(float, float, true) : CUSOAP_DENSE(float, float, false);
     else if (odt == CUSOAP_DT_F32) rc = vec ? CUSOAP_DENSE(float, double, true) : CUSOAP_DENSE(float, double, false);
     else rc = vec ? CUSOAP_DENSE(double, double, true) : CUSOAP_DENSE(double, double, false);
 #undef CUSOAP_DENSE
