@@ -7,9 +7,10 @@
 //            (counting sort), then one wavefront per center scans the 27
 //            surrounding cells and counts edges with a ballot
 //   scan   : exclusive scan of the counts -> CSR offsets
-//   fill   : one workgroup per center gathers its edges into LDS and
-//            bitonic-sorts them by ext index, which is the order of the
-//            reference's `nonzero` over (center, ext)
+//   fill   : one wavefront per center (one workgroup per center above 1024
+//            edges) gathers its edges into LDS and stores each at its rank
+//            among the center's ext indices (distinct keys), which is the order
+//            of the reference's `nonzero` over (center, ext)
 // The distance test restates the reference formula
 //   d^2 = ((-2 c.n) + |c|^2) + |n|^2 <= cutoff^2
 // in float64 with every operation rounded separately (no fused multiply-add), so
@@ -418,6 +419,60 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
     }
 }
 
+// Same list as fill_kernel, one wavefront per center (4 centers per workgroup) for
+// centers with at most CAP neighbours: the passing candidates are compacted with
+// ballot + mbcnt (no LDS atomics), and the rank placement runs on the wavefront's own
+// keys with no workgroup barrier.
+template <int CAP>
+__global__ __launch_bounds__(256) void fill_wave_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
+                                                        int n_centers, double cutoff2, int drop_self,
+                                                        const int64_t* __restrict__ cell_start,
+                                                        const double4* __restrict__ binned,
+                                                        const int32_t* __restrict__ binned_ext,
+                                                        const int64_t* __restrict__ offsets,
+                                                        int32_t* __restrict__ nbr) {
+    __shared__ __align__(16) int32_t keys_all[4][CAP + 4];
+    const int wq = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + wq;
+    if (c >= n_centers) return;     // whole wavefront; no workgroup barrier below
+    int32_t* keys = keys_all[wq];
+    const Grid g = make_grid(plan);
+    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
+    const double cn = sqnorm_rn(cx, cy, cz);
+    const CandRows cr = cand_rows(g, center_cells(g, cx, cy, cz), cell_start);
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int64_t s0 = cr.start[i];
+        const int len = cr.len[i];
+        for (int k0 = 0; k0 < len; k0 += 64) {
+            const int k = k0 + lane;
+            const bool pass = k < len && edge_test(cx, cy, cz, cn, binned[s0 + k], cutoff2, drop_self);
+            const uint64_t m = __ballot(pass);
+            const int slot = n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (pass && slot < CAP) keys[slot] = binned_ext[s0 + k];
+            n += __popcll(m);
+        }
+    }
+    n = n < CAP ? n : CAP;
+    const int n4 = (n + 3) & ~3;
+    if (lane < n4 - n) keys[n + lane] = INT_MAX;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t base = offsets[c];
+    for (int i = lane; i < n; i += 64) {
+        const int k = keys[i];
+        int rank = 0;
+        for (int j = 0; j < n4; j += 4) {
+            const int4 q = *(const int4*)(keys + j);
+            rank += (q.x < k) + (q.y < k) + (q.z < k) + (q.w < k);
+        }
+        nbr[base + rank] = k;
+    }
+}
+
 }  // namespace
 
 int nl_plan_impl(const double* centers, int32_t n_centers, int32_t n_atoms, int32_t n_img, double cutoff,
@@ -520,8 +575,9 @@ int nl_fill_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_atom
     const double cutoff2 = cutoff * cutoff;
     if (max_count <= 1024) {
         prof_begin("nl_fill", st);
-        fill_kernel<1024><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self, w.cell_start,
-                                                     w.binned, w.binned_ext, offsets, nbr);
+        fill_wave_kernel<1024><<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
+                                                                    w.cell_start, w.binned, w.binned_ext, offsets,
+                                                                    nbr);
         prof_end("nl_fill", st);
     } else if (max_count <= 4096) {
         prof_begin("nl_fill", st);
