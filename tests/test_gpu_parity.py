@@ -84,6 +84,19 @@ def test_nl_bit_exact_3000_and_coordinate_centers():
     np.testing.assert_array_equal(nbr.cpu().numpy(), ni)
 
 
+def test_nl_bit_exact_over_1024_neighbours():
+    """Centers with more than 1024 edges take the workgroup-per-center fill."""
+    s = systems.water_cluster_3000(seed=3)
+    o, g = pair(dict(W300, r_cut=20.0))
+    say("nl cluster3000 r_cut=20")
+    off, nbr = g.neighbor_list(s, drop_self=True)
+    counts = np.diff(off.cpu().numpy())
+    assert counts.max() > 1024
+    ci, ni, _, _ = o.neighbour_list(s["positions"], s["atomic_numbers"], s["positions"], drop_self=True)
+    np.testing.assert_array_equal(counts, np.bincount(ci, minlength=3000))
+    np.testing.assert_array_equal(nbr.cpu().numpy(), ni)
+
+
 # ---- descriptors ------------------------------------------------------------------
 
 @pytest.mark.parametrize("average", ["off", "inner", "outer", "cc"])
