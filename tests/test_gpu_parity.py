@@ -236,16 +236,18 @@ def test_jacobian_general_periodic_weighting_attach():
 def test_jacobian_mixed_f32_rows():
     """Mixed precision with float32 output: float32 edge records and rows, against float64."""
     s = systems.water_cluster_300(seed=4)
-    o = so.OracleSOAP(**W300)
-    g = SOAP(device="cuda:0", dtype="float32", precision="mixed", _betas=o.betas, **W300)
     cen = list(range(0, 300, 7))
-    say("jacobian mixed f32 cluster")
-    J, d = g.derivatives(s, centers=cen)
-    Jr, dr = o.derivatives(s, centers=cen)
-    assert J.dtype == torch.float32
-    say("  err", colerr(J, Jr))
-    assert colerr(J, Jr) < 1e-5
-    assert colerr(d, dr) < 1e-5
+    for average in ("off", "cc"):
+        p = dict(W300, average=average)
+        o = so.OracleSOAP(**p)
+        g = SOAP(device="cuda:0", dtype="float32", precision="mixed", _betas=o.betas, **p)
+        say("jacobian mixed f32 cluster", average)
+        J, d = g.derivatives(s, centers=cen)
+        Jr, dr = o.derivatives(s, centers=cen)
+        assert J.dtype == torch.float32
+        say("  err", colerr(J, Jr))
+        assert colerr(J, Jr) < 1e-5
+        assert colerr(d, dr) < 1e-5
     # periodic images sharing one row, the weighting slope and the attach row
     s = systems.water((2, 2, 2), spacing=3.0, seed=3, cell_length=6.0)
     p = dict(r_cut=4.0, n_max=3, l_max=3, sigma=0.5, species=[1, 8], periodic=True,
