@@ -5,12 +5,14 @@
 //   plan   : bounding box of the centers -> uniform grid of cells >= cutoff
 //   count  : bin every image atom (ext = img * N + atom) that falls in the grid
 //            (counting sort), then one wavefront per center scans the 27
-//            surrounding cells and counts edges with a ballot
+//            surrounding cells, counts edges with a ballot and, up to 384 edges,
+//            stores them in order in the workspace (each at its rank among the
+//            center's ext indices, which are distinct): the order of the
+//            reference's `nonzero` over (center, ext)
 //   scan   : exclusive scan of the counts -> CSR offsets
-//   fill   : one wavefront per center (one workgroup per center above 256
-//            edges) gathers its edges into LDS and stores each at its rank
-//            among the center's ext indices (distinct keys), which is the order
-//            of the reference's `nonzero` over (center, ext)
+//   fill   : a copy of the sorted rows to the CSR ranges; a center with more
+//            than 384 edges is searched again by a workgroup of its own, which
+//            places its edges by rank
 // The distance test restates the reference formula
 //   d^2 = ((-2 c.n) + |c|^2) + |n|^2 <= cutoff^2
 // in float64 with every operation rounded separately (no fused multiply-add), so
@@ -67,11 +69,17 @@ struct NLWs {
     double4* binned;       // (n_ext)  x, y, z, |p|^2
     int32_t* binned_ext;   // (n_ext)
     int32_t* counts;       // (n_centers)
+    int32_t* sorted;       // (n_centers, kSortCap) sorted edges of centers with <= kSortCap edges
     int64_t* partials;     // scan partials
     int32_t* max_count;    // (1)
     size_t bytes;
 };
 
+// the count pass also sorts the edges of every center with at most this many into the
+// workspace, so that the fill pass is a copy when no center has more (the 98k-atom
+// water box at r_cut 6 + 1.86 padding has at most ~240 edges per center; 1.5 KiB of
+// workspace per center)
+constexpr int kSortCap = 384;
 constexpr int kScanBlock = 1024;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanBlock * kScanItems;
@@ -92,6 +100,7 @@ NLWs carve(void* base, int64_t n_ext, int32_t n_cells, int32_t n_centers) {
     w.binned = (double4*)take(sizeof(double4) * n_ext);
     w.binned_ext = (int32_t*)take(sizeof(int32_t) * n_ext);
     w.counts = (int32_t*)take(sizeof(int32_t) * (n_centers + 1));
+    w.sorted = (int32_t*)take(sizeof(int32_t) * (size_t)kSortCap * n_centers);
     int64_t mx = std::max<int64_t>(n_cells + 1, n_centers + 1);
     w.partials = (int64_t*)take(sizeof(int64_t) * (scan_partials(mx) + kScanTile));
     w.max_count = (int32_t*)take(sizeof(int32_t) * 4);
@@ -328,31 +337,6 @@ __device__ __forceinline__ CandRows cand_rows(const Grid& g, const CenterCells& 
     return cr;
 }
 
-// one wavefront per center; the lanes stride through each candidate cell row
-__global__ __launch_bounds__(256) void count_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
-                                                    int n_centers, double cutoff2, int drop_self,
-                                                    const int64_t* __restrict__ cell_start,
-                                                    const double4* __restrict__ binned,
-                                                    int32_t* __restrict__ counts, int32_t* __restrict__ max_count) {
-    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (c >= n_centers) return;
-    const Grid g = make_grid(plan);
-    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
-    const double cn = sqnorm_rn(cx, cy, cz);
-    const CandRows cr = cand_rows(g, center_cells(g, cx, cy, cz), cell_start);
-    // row by row (uniform bounds): no per-candidate search for the row of a flattened index
-    int cnt = 0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const double4* row = binned + cr.start[i];
-        for (int k = lane; k < cr.len[i]; k += 64) cnt += edge_test(cx, cy, cz, cn, row[k], cutoff2, drop_self) ? 1 : 0;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-    if (lane == 0) counts[c] = cnt;
-}
-
 // largest per-center count: one atomic per workgroup (a per-center atomic on one
 // address serialises, ~1 ms per 10^5 centers)
 __global__ __launch_bounds__(256) void max_count_kernel(const int32_t* __restrict__ counts, int n,
@@ -374,11 +358,22 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
                                                    const double4* __restrict__ binned,
                                                    const int32_t* __restrict__ binned_ext,
                                                    const int64_t* __restrict__ offsets,
+                                                   const int32_t* __restrict__ sorted,
                                                    int32_t* __restrict__ nbr) {
     __shared__ __align__(16) int32_t keys[CAP + 4];
     __shared__ int nkeys;
     const int c = blockIdx.x;
     if (c >= n_centers) return;
+    {
+        // a center with at most kSortCap edges: the row the count pass sorted
+        const int64_t b0 = offsets[c];
+        const int cnt = (int)(offsets[c + 1] - b0);
+        if (cnt <= kSortCap) {
+            const int32_t* src = sorted + (size_t)c * kSortCap;
+            for (int i = threadIdx.x; i < cnt; i += blockDim.x) nbr[b0 + i] = src[i];
+            return;
+        }
+    }
     if (threadIdx.x == 0) nkeys = 0;
     __syncthreads();
     const Grid g = make_grid(plan);
@@ -419,18 +414,18 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
     }
 }
 
-// Same list as fill_kernel, one wavefront per center (4 centers per workgroup) when
-// no center has more than CAP neighbours: the passing candidates are compacted with
-// ballot + mbcnt (no LDS atomics), and the rank placement runs on the wavefront's own
-// keys with no workgroup barrier.
+// Count pass, one wavefront per center (4 centers per workgroup): counts the
+// center's edges and, when there are at most CAP, also puts them in order (the list
+// fill_kernel makes) in the center's row of `sorted`.  The passing candidates are
+// compacted with ballot + mbcnt (no LDS atomics), and the rank placement runs on the
+// wavefront's own keys with no workgroup barrier.
 template <int CAP>
-__global__ __launch_bounds__(256) void fill_wave_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
-                                                        int n_centers, double cutoff2, int drop_self,
-                                                        const int64_t* __restrict__ cell_start,
-                                                        const double4* __restrict__ binned,
-                                                        const int32_t* __restrict__ binned_ext,
-                                                        const int64_t* __restrict__ offsets,
-                                                        int32_t* __restrict__ nbr) {
+__global__ __launch_bounds__(256) void count_sort_kernel(cusoap_nl_plan_t plan, const double* __restrict__ centers,
+                                                         int n_centers, double cutoff2, int drop_self,
+                                                         const int64_t* __restrict__ cell_start,
+                                                         const double4* __restrict__ binned,
+                                                         const int32_t* __restrict__ binned_ext,
+                                                         int32_t* __restrict__ counts, int32_t* __restrict__ sorted) {
     __shared__ __align__(16) int32_t keys_all[4][CAP + 4];
     const int wq = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int c = blockIdx.x * 4 + wq;
@@ -455,13 +450,14 @@ __global__ __launch_bounds__(256) void fill_wave_kernel(cusoap_nl_plan_t plan, c
             n += __popcll(m);
         }
     }
-    n = n < CAP ? n : CAP;
+    if (lane == 0) counts[c] = n;
+    if (n > CAP) return;            // the fill pass searches this center again
     const int n4 = (n + 3) & ~3;
     if (lane < n4 - n) keys[n + lane] = INT_MAX;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int64_t base = offsets[c];
+    int32_t* dst = sorted + (size_t)c * CAP;
     for (int i = lane; i < n; i += 64) {
         const int k = keys[i];
         int rank = 0;
@@ -469,8 +465,21 @@ __global__ __launch_bounds__(256) void fill_wave_kernel(cusoap_nl_plan_t plan, c
             const int4 q = *(const int4*)(keys + j);
             rank += (q.x < k) + (q.y < k) + (q.z < k) + (q.w < k);
         }
-        nbr[base + rank] = k;
+        dst[rank] = k;
     }
+}
+
+// Fill pass when every center has at most kSortCap edges: one wavefront per center
+// copies its sorted row to its CSR range.
+__global__ __launch_bounds__(256) void copy_sorted_kernel(int n_centers, const int32_t* __restrict__ sorted,
+                                                          const int64_t* __restrict__ offsets,
+                                                          int32_t* __restrict__ nbr) {
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= n_centers) return;
+    const int64_t b = offsets[c];
+    const int cnt = (int)(offsets[c + 1] - b);
+    const int32_t* src = sorted + (size_t)c * kSortCap;
+    for (int i = lane; i < cnt; i += 64) nbr[b + i] = src[i];
 }
 
 }  // namespace
@@ -546,8 +555,9 @@ int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_ato
     const double cutoff2 = cutoff * cutoff;
     if (n_centers > 0) {
         prof_begin("nl_count", st);
-        count_kernel<<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
-                                                          w.cell_start, w.binned, w.counts, w.max_count);
+        count_sort_kernel<kSortCap><<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2,
+                                                                          drop_self, w.cell_start, w.binned,
+                                                                          w.binned_ext, w.counts, w.sorted);
         max_count_kernel<<<std::min(256, (n_centers + 255) / 256), 256, 0, st>>>(w.counts, n_centers, w.max_count);
         prof_end("nl_count", st);
         CUSOAP_LAUNCH_CHECK();
@@ -573,30 +583,29 @@ int nl_fill_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_atom
     if (ws_bytes < w.bytes) return fail(CUSOAP_ERR_WORKSPACE, "nl_fill: workspace too small");
     if (n_centers == 0 || max_count == 0) return CUSOAP_OK;
     const double cutoff2 = cutoff * cutoff;
-    // a wavefront per center up to 256 edges; above that the rank placement (each key
-    // compared with every key of its center) is shared by a whole workgroup
-    // ((H2O)_1000 at r_cut 10, ~450 edges per center: 0.073 ms by workgroup, 0.113 ms
-    // by wavefront)
-    if (max_count <= 256) {
+    // up to kSortCap edges per center the count pass has sorted them already; above
+    // that the search runs again for those centers, with a whole workgroup per center
+    // sharing the rank placement (each key compared with every key of its center), which one wavefront
+    // does too slowly at ~450 edges ((H2O)_1000 at r_cut 10: 0.073 ms by workgroup,
+    // 0.113 ms by wavefront)
+    if (max_count <= kSortCap) {
         prof_begin("nl_fill", st);
-        fill_wave_kernel<256><<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
-                                                                   w.cell_start, w.binned, w.binned_ext, offsets,
-                                                                   nbr);
+        copy_sorted_kernel<<<(n_centers + 3) / 4, 256, 0, st>>>(n_centers, w.sorted, offsets, nbr);
         prof_end("nl_fill", st);
     } else if (max_count <= 1024) {
         prof_begin("nl_fill", st);
         fill_kernel<1024><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self, w.cell_start,
-                                                     w.binned, w.binned_ext, offsets, nbr);
+                                                     w.binned, w.binned_ext, offsets, w.sorted, nbr);
         prof_end("nl_fill", st);
     } else if (max_count <= 4096) {
         prof_begin("nl_fill", st);
         fill_kernel<4096><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self, w.cell_start,
-                                                     w.binned, w.binned_ext, offsets, nbr);
+                                                     w.binned, w.binned_ext, offsets, w.sorted, nbr);
         prof_end("nl_fill", st);
     } else if (max_count <= 16384) {
         prof_begin("nl_fill", st);
         fill_kernel<16384><<<n_centers, 256, 0, st>>>(*plan, centers, n_centers, cutoff2, drop_self,
-                                                      w.cell_start, w.binned, w.binned_ext, offsets, nbr);
+                                                      w.cell_start, w.binned, w.binned_ext, offsets, w.sorted, nbr);
         prof_end("nl_fill", st);
     } else {
         return fail(CUSOAP_ERR_CAPACITY, "nl_fill: a center has more than 16384 neighbours within the cutoff");

@@ -126,7 +126,9 @@ int cusoap_nl_count(const cusoap_nl_plan_t* plan, const double* positions, int32
                     const double* shifts, int32_t n_img, const double* centers, int32_t n_centers,
                     double cutoff, int32_t drop_self, void* ws, size_t ws_bytes,
                     int64_t* offsets, int64_t* total_host, int32_t* max_count_host, void* stream);
-/* Writes the sorted ext indices (int32) of every center's edges. */
+/* Writes the sorted ext indices (int32) of every center's edges.  `ws` must be the
+ * workspace that cusoap_nl_count filled for the same plan and centers (it holds the
+ * binned images and the rows the count pass already sorted). */
 int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_t n_atoms,
                    const double* shifts, int32_t n_img, const double* centers, int32_t n_centers,
                    double cutoff, int32_t drop_self, void* ws, size_t ws_bytes,

@@ -86,8 +86,8 @@ def test_nl_bit_exact_3000_and_coordinate_centers():
 
 def test_nl_bit_exact_over_1024_neighbours():
     """Centers with more than 1024 edges: the workgroup-per-center fill with 4096 slots
-    (the 3,000-atom test at r_cut 10 takes the 1024-slot one, the 300-atom tests the
-    wavefront-per-center one)."""
+    (the 3,000-atom test at r_cut 10 takes the 1024-slot one; the 300-atom tests take
+    the rows that the count pass sorted)."""
     s = systems.water_cluster_3000(seed=3)
     o, g = pair(dict(W300, r_cut=20.0))
     say("nl cluster3000 r_cut=20")
