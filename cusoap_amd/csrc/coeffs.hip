@@ -163,6 +163,11 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const int64_t e0 = offsets[c], e1 = offsets[c + 1];
     const int npair = (n + 1) / 2;
     const int nslot_all = npair * NG;                 // (n pair, lm group) slots per species
+    // few slots (small n, l): the wavefront splits into `split` lane groups of `width`
+    // lanes, group h summing every split-th of the wavefront's positions (offset h);
+    // the groups' sums are added with cross-lane shuffles before the combine
+    const int split = nslot_all <= 16 ? 4 : (nslot_all <= 32 ? 2 : 1);
+    const int width = 64 / split, lgrp = lane / width;
     const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
     const int cslot = (RBF == kRbfTable && self_sp[c] >= 0) ? self_sp[c] : S;   // table slot of the center
     const double* g_gamma = gcon;
@@ -170,16 +175,21 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const double* g_beta = gcon + 2 * L1 * n;
     // table phase: this lane's (l, n) channel of the first 64; grid constants
     const int L1n = L1 * n, ch_l = lane / n, ch_n = lane - (lane / n) * n;
+    // at most 32 channels (small n, l): the two half-wavefronts take two edge positions
+    // at a time, lane & 31 being the channel
+    const bool ch_pair = L1n <= 32;
+    const int pc = lane & 31, pc_l = pc / n, pc_n = pc - (pc / n) * n, phalf = lane >> 5;
     const double tab_inv_h = RBF == kRbfTable ? b.rad_intervals / b.rad_rmax : 0.0;
     const TA tab_h = RBF == kRbfTable ? (TA)(b.rad_rmax / b.rad_intervals) : TA(0);
     __syncthreads();
 
     for (int s0 = 0; s0 < S; s0 += SB) {
         const int s1 = (s0 + SB < S) ? s0 + SB : S;
-        for (int q0 = 0; q0 < nslot_all; q0 += 64) {
+        for (int q0 = 0; q0 < nslot_all; q0 += width) {
             // ---- this lane's slot: n pair p (n0 = 2p, n1 = 2p+1) x lm group g of degree l
-            const int qs = q0 + lane;
+            const int qs = q0 + (lane - lgrp * width);
             const bool valid = qs < nslot_all;
+            const bool owner = valid && lgrp == 0;          // writes the slot's combined sum
             int n0 = 0, l = 0, lm0 = 0, cnt = 0, grp = 0;
             bool has1 = false;
             if (valid) {
@@ -275,6 +285,28 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                         const int4 rows = *(const int4*)(rq + k0);
                         const int rw[4] = {rows.x, ne > 1 ? rows.y : rows.x, ne > 2 ? rows.z : rows.x,
                                            ne > 3 ? rows.w : rows.x};
+                        if (ch_pair) {
+                            // half h: slots j = h and j = h + 2
+                            const bool on = pc < L1n;
+                            const int coff = on ? (pc_l * n + pc_n) * 2 : 0;
+                            const int ra = phalf ? rw[1] : rw[0], rb = phalf ? rw[3] : rw[2];
+                            const float* aa = b.rad_table + (uint32_t)(ra + coff);
+                            const float* ab = b.rad_table + (uint32_t)(rb + coff);
+                            const float2 va0 = *(const float2*)aa, va1 = *(const float2*)(aa + NS);
+                            const float2 vb0 = *(const float2*)ab, vb1 = *(const float2*)(ab + NS);
+                            if (on) {
+                                const int ja = k0 + phalf, jb = k0 + phalf + 2;
+                                const typename V2<TA>::type wa01 = *(const typename V2<TA>::type*)(wq + ja * 4);
+                                const typename V2<TA>::type wa23 = *(const typename V2<TA>::type*)(wq + ja * 4 + 2);
+                                const typename V2<TA>::type wb01 = *(const typename V2<TA>::type*)(wq + jb * 4);
+                                const typename V2<TA>::type wb23 = *(const typename V2<TA>::type*)(wq + jb * 4 + 2);
+                                Rs[(q + 4 * ja) * RS + pc_l * nP + pc_n] = wa01.x * (TA)va0.x + wa01.y * (TA)va0.y +
+                                                                          wa23.x * (TA)va1.x + wa23.y * (TA)va1.y;
+                                Rs[(q + 4 * jb) * RS + pc_l * nP + pc_n] = wb01.x * (TA)vb0.x + wb01.y * (TA)vb0.y +
+                                                                          wb23.x * (TA)vb1.x + wb23.y * (TA)vb1.y;
+                            }
+                            continue;
+                        }
                         for (int ch0 = 0; ch0 < L1n; ch0 += 64) {
                             const bool on = ch0 + lane < L1n;
                             int ll = ch_l, nn = ch_n;
@@ -377,21 +409,31 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                             }
                         };
                         // two positions per step so that their LDS reads are in flight together
-                        const int pe = beg[a4] + scnt[a4];
-                        int pp = beg[a4] + ((q - beg[a4]) & 3);
-                        for (; pp + 4 < pe; pp += 8) {
+                        const int pe = beg[a4] + scnt[a4], ps = 4 * split;
+                        int pp = beg[a4] + ((q - beg[a4]) & 3) + 4 * lgrp;
+                        for (; pp + ps < pe; pp += 2 * ps) {
                             fma_edge(pp);
-                            fma_edge(pp + 4);
+                            fma_edge(pp + ps);
                         }
                         if (pp < pe) fma_edge(pp);
                     }
                 }
                 __syncthreads();
             }
+            // ---- lane groups of one slot: fixed-order butterfly (every lane of the wavefront
+            // takes part; lanes of invalid slots carry zeros)
+            for (int o = 32; o >= width; o >>= 1) {
+#pragma unroll
+                for (int a4 = 0; a4 < SB; ++a4)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[a4][h][j] += __shfl_xor(acc[a4][h][j], o, 64);
+            }
             // ---- combine the four wavefronts' partial sums in a fixed order (LDS, aliases the tile)
             TA* part = Rs;                                  // [SB species][n][L2]
             for (int w2 = 0; w2 < 4; ++w2) {
-                if (wv == w2 && valid) {
+                if (wv == w2 && owner) {
 #pragma unroll
                     for (int a4 = 0; a4 < SB; ++a4) {
                         if (s0 + a4 >= s1) continue;
@@ -410,7 +452,7 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 __syncthreads();
             }
             // ---- write this pass's outputs (raw primitive sums when beta is applied per center)
-            if (wv == 0 && valid) {
+            if (wv == 0 && owner) {
                 for (int a4 = 0; a4 < s1 - s0; ++a4)
                     for (int h = 0; h < 2; ++h) {
                         if (h == 1 && !has1) continue;
