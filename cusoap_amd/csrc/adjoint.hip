@@ -296,7 +296,11 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     constexpr bool MIXED = RAD == kRadMixed;
     constexpr bool FIXN = NN > 0;
     constexpr int NR = FIXN ? NN : 1;
-    constexpr int NQ = FIXN && RAD == kRadTable ? (NN + 1) / 2 : 1;
+    // node rows in registers: (value, slope) pairs of two n per float4, or of one n per
+    // float2 when n is odd (rows of 2n floats are then 8-byte aligned only)
+    constexpr bool ODDN = (NN & 1) != 0;
+    using RowV = typename std::conditional<ODDN, float2, float4>::type;
+    constexpr int NQ = FIXN && RAD == kRadTable ? (ODDN ? NN : NN / 2) : 1;
     const int n = FIXN ? NN : b.n_max, S = b.n_species;
     extern __shared__ __align__(16) unsigned char smem[];
     double* qd = (double*)smem;                             // [n][NT]   (MIXED, runtime n) primitives
@@ -411,12 +415,12 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                 v11 *= tw;
             }
             TM rr0[NR], rr1[NR];                                 // radial vectors of degree l (FIXN)
-            // register table form: node rows of the next degree, (value, slope) pairs of two n per float4
-            float4 nx0[NQ], nx1[NQ];
+            // register table form: node rows of the next degree
+            RowV nx0[NQ], nx1[NQ];
             auto load_rows = [&](int l) {
                 if constexpr (FIXN && RAD == kRadTable) {
-                    const float4* a0 = (const float4*)(tb + l * 2 * NN);
-                    const float4* a1 = (const float4*)(tb + (L1 + l) * 2 * NN);
+                    const RowV* a0 = (const RowV*)(tb + l * 2 * NN);
+                    const RowV* a1 = (const RowV*)(tb + (L1 + l) * 2 * NN);
 #pragma unroll
                     for (int k = 0; k < NQ; ++k) {
                         nx0[k] = a0[k];
@@ -431,7 +435,7 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                     if (RAD == kRadTable) {
                         // this degree's rows were loaded one degree ahead; issue the next degree's
                         // loads before using them, so their latency overlaps this degree's work
-                        float4 c0[NQ], c1[NQ];
+                        RowV c0[NQ], c1[NQ];
 #pragma unroll
                         for (int k = 0; k < NQ; ++k) {
                             c0[k] = nx0[k];
@@ -440,9 +444,14 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                         if (l + 1 < L1) load_rows(l + 1);
 #pragma unroll
                         for (int k = 0; k < NN; ++k) {
-                            const float4 q0 = c0[k >> 1], q1 = c1[k >> 1];
-                            const TM b0 = (TM)((k & 1) ? q0.z : q0.x), d0 = (TM)((k & 1) ? q0.w : q0.y);
-                            const TM b1 = (TM)((k & 1) ? q1.z : q1.x), d1 = (TM)((k & 1) ? q1.w : q1.y);
+                            TM b0, d0, b1, d1;
+                            if constexpr (ODDN) {
+                                b0 = (TM)c0[k].x; d0 = (TM)c0[k].y; b1 = (TM)c1[k].x; d1 = (TM)c1[k].y;
+                            } else {
+                                const float4 q0 = c0[k >> 1], q1 = c1[k >> 1];
+                                b0 = (TM)((k & 1) ? q0.z : q0.x); d0 = (TM)((k & 1) ? q0.w : q0.y);
+                                b1 = (TM)((k & 1) ? q1.z : q1.x); d1 = (TM)((k & 1) ? q1.w : q1.y);
+                            }
                             rr0[k] = v00 * b0 + v10 * d0 + v01 * b1 + v11 * d1;
                             rr1[k] = c00 * b0 + c10 * d0 + c01 * b1 + c11 * d1;
                         }
@@ -726,11 +735,12 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
                      const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
                      double* grad, hipStream_t st) {
 #define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, st
-    const bool n8 = b.n_max == 8;         // registers for the radial vectors (float table form)
     if (b.rad_table) {
         if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable, 0>(CUSOAP_VJP_ARGS);
-        return n8 ? launch_vjp_gto<LMAX, float, kRadTable, 8>(CUSOAP_VJP_ARGS)
-                  : launch_vjp_gto<LMAX, float, kRadTable, 0>(CUSOAP_VJP_ARGS);
+        // registers for the radial vectors (float table form) at n_max 8 and 7
+        if (b.n_max == 8) return launch_vjp_gto<LMAX, float, kRadTable, 8>(CUSOAP_VJP_ARGS);
+        if (b.n_max == 7) return launch_vjp_gto<LMAX, float, kRadTable, 7>(CUSOAP_VJP_ARGS);
+        return launch_vjp_gto<LMAX, float, kRadTable, 0>(CUSOAP_VJP_ARGS);
     }
     if (b.rbf == CUSOAP_RBF_GTO) {
         if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadPrim, 0>(CUSOAP_VJP_ARGS);

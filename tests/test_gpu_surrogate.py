@@ -187,10 +187,12 @@ def test_training_meets_the_gate(learned):
     assert e["value"].max() <= 2e-5 and e["slope"].max() <= 1e-4
 
 
-def test_exact_table_interpolation_matches_oracle():
-    """The table path with exact node values isolates the interpolation from the network."""
+@pytest.mark.parametrize("n_max,l_max", [(8, 6), (7, 3), (5, 4)])
+def test_exact_table_interpolation_matches_oracle(n_max, l_max):
+    """The table path with exact node values isolates the interpolation from the network
+    (edge pass: radial vectors in registers at n_max 8 and 7, in LDS otherwise)."""
     s = systems.water_box(5, seed=4)
-    p = dict(WB, periodic=True)
+    p = dict(WB, periodic=True, n_max=n_max, l_max=l_max)
     o = so.OracleSOAP(**p)
     w = np.random.default_rng(5).normal(size=o.get_number_of_features())
     Fr = o.vjp(s, w[None, :].repeat(len(s["positions"]), 0))
@@ -199,7 +201,7 @@ def test_exact_table_interpolation_matches_oracle():
         g = SOAP(device=DEV, dtype="float64", precision=prec, radial="table", _betas=o.betas, **p)
         d, F = g.descriptor_and_vjp(s, torch.tensor(w))
         ed, ef = colerr(d, ref), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
-        say(f"exact table ({prec}): desc err {ed:.2e}, force err {ef:.2e}")
+        say(f"exact table n{n_max} l{l_max} ({prec}): desc err {ed:.2e}, force err {ef:.2e}")
         assert ed < tol[0] and ef < tol[1]
 
 
