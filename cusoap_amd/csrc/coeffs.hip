@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     // few slots (small n, l): the wavefront splits into `split` lane groups of `width`
     // lanes, group h summing every split-th of the wavefront's positions (offset h);
     // the groups' sums are added with cross-lane shuffles before the combine
-    const int split = nslot_all <= 16 ? 4 : (nslot_all <= 32 ? 2 : 1);
+    // (l_max <= 4 only, as for the channel pairs)
+    const int split = LMAX > 4 ? 1 : (nslot_all <= 16 ? 4 : (nslot_all <= 32 ? 2 : 1));
     const int width = 64 / split, lgrp = lane / width;
     const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
     const int cslot = (RBF == kRbfTable && self_sp[c] >= 0) ? self_sp[c] : S;   // table slot of the center
@@ -177,7 +178,8 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const int L1n = L1 * n, ch_l = lane / n, ch_n = lane - (lane / n) * n;
     // at most 32 channels (small n, l): the two half-wavefronts take two edge positions
     // at a time, lane & 31 being the channel
-    const bool ch_pair = L1n <= 32;
+    // (compiled in for l_max <= 4 only: larger l_max keeps the plain code)
+    const bool ch_pair = LMAX <= 4 && L1n <= 32;
     const int pc = lane & 31, pc_l = pc / n, pc_n = pc - (pc / n) * n, phalf = lane >> 5;
     const double tab_inv_h = RBF == kRbfTable ? b.rad_intervals / b.rad_rmax : 0.0;
     const TA tab_h = RBF == kRbfTable ? (TA)(b.rad_rmax / b.rad_intervals) : TA(0);
