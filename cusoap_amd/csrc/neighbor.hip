@@ -21,6 +21,8 @@
 #include <climits>
 #include <cmath>
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace cusoap {
@@ -458,14 +460,35 @@ __global__ __launch_bounds__(256) void count_sort_kernel(cusoap_nl_plan_t plan, 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int32_t* dst = sorted + (size_t)c * CAP;
-    for (int i = lane; i < n; i += 64) {
-        const int k = keys[i];
-        int rank = 0;
+    // each lane holds its keys i = lane + 64 t in registers and ranks all of them in one
+    // pass over the keys (one 16-byte LDS read serves every key of the lane)
+    auto place = [&](auto kpl_c) {
+        constexpr int KPL = decltype(kpl_c)::value;
+        int kk[KPL], rk[KPL];
+#pragma unroll
+        for (int t = 0; t < KPL; ++t) {
+            const int i = lane + 64 * t;
+            kk[t] = i < n ? keys[i] : INT_MAX;
+            rk[t] = 0;
+        }
         for (int j = 0; j < n4; j += 4) {
             const int4 q = *(const int4*)(keys + j);
-            rank += (q.x < k) + (q.y < k) + (q.z < k) + (q.w < k);
+#pragma unroll
+            for (int t = 0; t < KPL; ++t) rk[t] += (q.x < kk[t]) + (q.y < kk[t]) + (q.z < kk[t]) + (q.w < kk[t]);
         }
-        dst[rank] = k;
+#pragma unroll
+        for (int t = 0; t < KPL; ++t)
+            if (lane + 64 * t < n) dst[rk[t]] = kk[t];
+    };
+    static_assert(CAP <= 6 * 64, "count_sort_kernel: at most 6 keys per lane");
+    switch ((n + 63) >> 6) {
+        case 0: break;
+        case 1: place(std::integral_constant<int, 1>()); break;
+        case 2: place(std::integral_constant<int, 2>()); break;
+        case 3: place(std::integral_constant<int, 3>()); break;
+        case 4: place(std::integral_constant<int, 4>()); break;
+        case 5: place(std::integral_constant<int, 5>()); break;
+        default: place(std::integral_constant<int, 6>()); break;
     }
 }
 
