@@ -737,10 +737,15 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
 #define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, st
     if (b.rad_table) {
         if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable, 0>(CUSOAP_VJP_ARGS);
-        // registers for the radial vectors (float table form) at n_max 8 and 7
-        if (b.n_max == 8) return launch_vjp_gto<LMAX, float, kRadTable, 8>(CUSOAP_VJP_ARGS);
-        if (b.n_max == 7) return launch_vjp_gto<LMAX, float, kRadTable, 7>(CUSOAP_VJP_ARGS);
-        return launch_vjp_gto<LMAX, float, kRadTable, 0>(CUSOAP_VJP_ARGS);
+        // registers for the radial vectors (float table form) at n_max 4..8
+        switch (b.n_max) {
+            case 8: return launch_vjp_gto<LMAX, float, kRadTable, 8>(CUSOAP_VJP_ARGS);
+            case 7: return launch_vjp_gto<LMAX, float, kRadTable, 7>(CUSOAP_VJP_ARGS);
+            case 6: return launch_vjp_gto<LMAX, float, kRadTable, 6>(CUSOAP_VJP_ARGS);
+            case 5: return launch_vjp_gto<LMAX, float, kRadTable, 5>(CUSOAP_VJP_ARGS);
+            case 4: return launch_vjp_gto<LMAX, float, kRadTable, 4>(CUSOAP_VJP_ARGS);
+            default: return launch_vjp_gto<LMAX, float, kRadTable, 0>(CUSOAP_VJP_ARGS);
+        }
     }
     if (b.rbf == CUSOAP_RBF_GTO) {
         if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadPrim, 0>(CUSOAP_VJP_ARGS);
