@@ -432,8 +432,49 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
 #pragma unroll
                         for (int j = 0; j < 4; ++j) acc[a4][h][j] += __shfl_xor(acc[a4][h][j], o, 64);
             }
-            // ---- combine the four wavefronts' partial sums in a fixed order (LDS, aliases the tile)
+            // ---- combine the four wavefronts' partial sums in a fixed order (LDS, aliases the tile).
+            // When four [SB][n][L2] blocks fit in the tile region, each wavefront stores its own
+            // block and one barrier precedes the sum ((w0 + w1) + w2) + w3; otherwise the
+            // wavefronts add into one block in turn (the same order, four barriers).
             TA* part = Rs;                                  // [SB species][n][L2]
+            const int psz = SB * n * L2;
+            const int region = (kTE * RS > (S > 4 ? S : 4) * n * L2) ? kTE * RS : (S > 4 ? S : 4) * n * L2;
+            const bool blocks4 = 4 * psz <= region;
+            if (blocks4) {
+                if (owner) {
+                    TA* mine = part + wv * psz;
+#pragma unroll
+                    for (int a4 = 0; a4 < SB; ++a4) {
+                        if (s0 + a4 >= s1) continue;
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            if (h == 1 && !has1) continue;
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                if (j >= cnt) continue;
+                                mine[(a4 * n + n0 + h) * L2 + lm0 + j] = acc[a4][h][j];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                if (wv == 0 && owner) {
+                    for (int a4 = 0; a4 < s1 - s0; ++a4)
+                        for (int h = 0; h < 2; ++h) {
+                            if (h == 1 && !has1) continue;
+                            const int nn = n0 + h;
+                            TA* dst = out + (((size_t)c * S + s0 + a4) * n + nn) * L2 + lm0;
+                            for (int j = 0; j < cnt; ++j) {
+                                const int o = (a4 * n + nn) * L2 + lm0 + j;
+                                TA v = ((part[o] + part[psz + o]) + part[2 * psz + o]) + part[3 * psz + o];
+                                if (!kBetaPerCenter && s0 + a4 == self_s && lm0 + j == 0) v += (TA)b.self_l0[nn];
+                                dst[j] = v;
+                            }
+                        }
+                }
+                __syncthreads();
+                continue;
+            }
             for (int w2 = 0; w2 < 4; ++w2) {
                 if (wv == w2 && owner) {
 #pragma unroll
