@@ -252,11 +252,12 @@ constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
 // lanes busy with groups of 8; the register-table form is bound by the latency of
 // its table reads and measured fastest with one center per workgroup
 constexpr int kVjpGroup = 8;
-// register-table form: 4 neighbouring centers per workgroup, their force contributions
+// register-table form: 8 neighbouring centers per workgroup, their force contributions
 // summed per distinct neighbour atom in an LDS hash table (float sums, linear probing)
 // before one float64 atomic per atom and component: the neighbourhoods of consecutive
-// centers overlap, so this cuts the global atomics (executed at the memory side) ~2x
-constexpr int kVjpHashGroup = 4;
+// centers overlap, so this cuts the global atomics (executed at the memory side).
+// 8 centers against 4: water box edge pass 2.12 -> 1.96 ms, 1M cluster 12.7 -> 11.1 ms
+constexpr int kVjpHashGroup = 8;
 constexpr int kVjpHashSlots = 1024;
 constexpr int kVjpHashProbe = 32;      // probes before an edge falls back to global atomics
 template <int NN>
