@@ -252,12 +252,15 @@ constexpr int kRadPrim = 0, kRadMixed = 1, kRadTable = 2;
 // lanes busy with groups of 8; the register-table form is bound by the latency of
 // its table reads and measured fastest with one center per workgroup
 constexpr int kVjpGroup = 8;
-// register-table form: 8 neighbouring centers per workgroup, their force contributions
+// register-table form: up to 16 neighbouring centers per workgroup, their force contributions
 // summed per distinct neighbour atom in an LDS hash table (float sums, linear probing)
 // before one float64 atomic per atom and component: the neighbourhoods of consecutive
 // centers overlap, so this cuts the global atomics (executed at the memory side).
-// 8 centers against 4: water box edge pass 2.12 -> 1.96 ms, 1M cluster 12.7 -> 11.1 ms
-constexpr int kVjpHashGroup = 8;
+// 8 centers against 4: water box edge pass 2.12 -> 1.96 ms, 1M cluster 12.7 -> 11.1 ms;
+// 16 (as many as 64 KB of coefficient rows allow): 1M cluster 11.1 -> 10.5 ms, water box
+// unchanged (its rows allow 16 too)
+constexpr int kVjpHashGroup = 16;
+constexpr int kVjpMaxGroup = kVjpGroup > kVjpHashGroup ? kVjpGroup : kVjpHashGroup;
 constexpr int kVjpHashSlots = 1024;
 constexpr int kVjpHashProbe = 32;      // probes before an edge falls back to global atomics
 template <int NN>
@@ -308,10 +311,10 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     TM* rad0 = (TM*)(qd + (MIXED && !FIXN ? n * NT : 0));   // [n][NT]   (runtime n)
     TM* rad1 = rad0 + (FIXN ? 0 : n * NT);                  // [n][NT]   (runtime n)
     TM* Ms = rad1 + (FIXN ? 0 : n * NT);                    // [ngrp][S][L2][n]
-    __shared__ int64_t goff[kVjpGroup + 1];                 // the group's edge offsets
-    __shared__ double gcen[kVjpGroup][3];
-    __shared__ int gslot[kVjpGroup];                        // table slot of each center
-    __shared__ double red[kVjpGroup][3];                    // center-attach sums
+    __shared__ int64_t goff[kVjpMaxGroup + 1];              // the group's edge offsets
+    __shared__ double gcen[kVjpMaxGroup][3];
+    __shared__ int gslot[kVjpMaxGroup];                        // table slot of each center
+    __shared__ double red[kVjpMaxGroup][3];                    // center-attach sums
     constexpr bool HASH = FIXN && RAD == kRadTable;
     __shared__ int hkey[HASH ? kVjpHashSlots : 1];          // neighbour atom of each slot or -1
     __shared__ float hval[HASH ? kVjpHashSlots : 1][3];     // its summed contributions
@@ -714,8 +717,11 @@ int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, cons
     const size_t cols = sizeof(TM) * (NN > 0 ? 0 : 2 * 256 * (size_t)b.n_max) +
                         (RAD == kRadMixed && NN == 0 ? sizeof(double) * 256 * (size_t)b.n_max : 0);
     const size_t per_bytes = sizeof(TM) * (size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1);
-    // as many centers per workgroup as fit in 64 KB of coefficient rows (at least one)
-    int ngrp = (int)std::min<size_t>(vjp_group<NN>(), std::max<size_t>(1, (64 * 1024 - std::min<size_t>(cols, 64 * 1024)) / per_bytes));
+    // as many centers per workgroup as fit in 64 KB of coefficient rows (at least one); the
+    // register-table form keeps them within 32 KB, so that three workgroups with their hash
+    // tables share a CU (H/C/N/O, 6.3 KB rows: 10 centers took 0.48 ms against 0.41 for 4)
+    const size_t rows_budget = NN > 0 ? 32 * 1024 : 64 * 1024;
+    int ngrp = (int)std::min<size_t>(vjp_group<NN>(), std::max<size_t>(1, (rows_budget - std::min<size_t>(cols, rows_budget)) / per_bytes));
     const size_t smem = cols + ngrp * per_bytes;
     if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: coefficient block too large for LDS");
     auto kern = vjp_gto_kernel<LMAX, TM, RAD, NN>;
