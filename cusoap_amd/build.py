@@ -17,6 +17,9 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcusoap_amd.so")
 SOURCES = ["capi.hip", "neighbor.hip", "coeffs.hip", "spectrum.hip", "jacobian.hip", "adjoint.hip", "surrogate.hip"]
+# compiled once per l_max value (-DCUSOAP_ONLY_L=l, that degree's kernels) plus once for the rest
+PER_DEGREE = ("coeffs.hip", "adjoint.hip")
+MAX_L = 9
 ARCH = os.environ.get("CUSOAP_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops: never emit packed FP32 VALU ops (v_pk_fma_f32 & co).
@@ -32,20 +35,34 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build cusoap_amd)")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
-    cmd = [hipcc(), *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def units():
+    """(source, extra flags, object name) of every compilation."""
+    out = []
+    for src in SOURCES:
+        out.append((src, [], src.replace(".hip", ".o")))
+        if src in PER_DEGREE:
+            out += [(src, [f"-DCUSOAP_ONLY_L={l}"], src.replace(".hip", f"_l{l}.o")) for l in range(MAX_L + 1)]
+    return out
+
+
+def _compile(unit) -> str:
+    src, flags, name = unit
+    obj = os.path.join(OBJ, name)
+    cmd = [hipcc(), *HIPFLAGS, *flags, "-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr[-6000:]}")
+        raise RuntimeError(f"hipcc failed for {name}:\n{res.stderr[-6000:]}")
     return obj
 
 
 def build(jobs: int | None = None, verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
+    todo = units()
+    jobs = jobs or min(len(todo), os.cpu_count() or 4, 16)
+    # the heaviest compilations first, so the pool ends together
+    todo.sort(key=lambda u: (u[0] not in PER_DEGREE, -int(u[1][0].split("=")[1]) if u[1] else 0))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, todo))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -835,24 +835,68 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
 
 }  // namespace
 
+// build.py compiles this file once per l_max value with -DCUSOAP_ONLY_L=<l> (the edge
+// kernels of one degree) and once without it (everything else, including the entry
+// points, which call the per-degree functions).
+#define CUSOAP_VJP_PARAMS const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp, \
+    const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,                     \
+    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M, double* grad,          \
+    hipStream_t st
+#define CUSOAP_VJP_DECL(L) int vjp_edges_l##L(CUSOAP_VJP_PARAMS);
+CUSOAP_VJP_DECL(0) CUSOAP_VJP_DECL(1) CUSOAP_VJP_DECL(2) CUSOAP_VJP_DECL(3) CUSOAP_VJP_DECL(4)
+CUSOAP_VJP_DECL(5) CUSOAP_VJP_DECL(6) CUSOAP_VJP_DECL(7) CUSOAP_VJP_DECL(8) CUSOAP_VJP_DECL(9)
+#undef CUSOAP_VJP_DECL
+
+#ifdef CUSOAP_ONLY_L
+#define CUSOAP_VJP_DEF(L)                                                                                        \
+    int vjp_edges_l##L(CUSOAP_VJP_PARAMS) {                                                                      \
+        return launch_vjp_edges<L>(b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach,     \
+                                   center_species, offsets, nbr, M, grad, st);                                   \
+    }
+#define CUSOAP_VJP_DEF2(L) CUSOAP_VJP_DEF(L)
+CUSOAP_VJP_DEF2(CUSOAP_ONLY_L)
+#else
 size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) { return vjp_m_bytes(b, n_centers) + vjp_w_bytes(b); }
 
-int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
-             const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
-             const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs, int coeffs_dtype,
-             const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast, int n_feat, void* ws,
-             size_t ws_bytes, double* grad, hipStream_t st) {
-    (void)n_img;
+namespace {
+
+// grad = 0 by a kernel rather than hipMemsetAsync (no memset node for callers that
+// capture HIP graphs)
+__global__ void zero_f64_kernel(double* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+int zero_f64(double* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return CUSOAP_OK;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    zero_f64_kernel<<<(int)blocks, 256, 0, st>>>(p, n);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
+// checks shared by both entry points; returns CUSOAP_OK or the error, and the precision
+// the kernels run in (the polynomial basis always runs in float64)
+int vjp_check(const cusoap_basis_t* b, int mode, int& prec, int n_centers, const int32_t* center_species,
+              const int32_t* pair_feature) {
     if (!b || b->n_max < 1 || b->n_max > kMaxN || b->l_max < 0 || b->l_max > kMaxL)
         return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: n_max must be 1..16 and l_max 0..9");
     if (mode == CUSOAP_MODE_PS && !pair_feature) return fail(CUSOAP_ERR_ARG, "vjp: pair_feature table missing");
     if (prec < 0 || prec > 2) return fail(CUSOAP_ERR_ARG, "vjp: unknown precision");
-    if (ws_bytes < vjp_ws_bytes(b, n_centers)) return fail(CUSOAP_ERR_WORKSPACE, "vjp: workspace too small");
-    CUSOAP_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * (size_t)n_atoms, st));
-    if (n_centers == 0) return CUSOAP_OK;
-    if (b->rad_table && (!center_species || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
+    if (n_centers < 0) return fail(CUSOAP_ERR_ARG, "vjp: negative center count");
+    if (n_centers > 0 && b->rad_table && (!center_species || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
         return fail(CUSOAP_ERR_ARG, "vjp: radial table needs center_species and a valid grid");
-    if (b->rbf != CUSOAP_RBF_GTO && !b->rad_table) prec = CUSOAP_PREC_F64;   // the polynomial path is float64
+    if (b->rbf != CUSOAP_RBF_GTO && !b->rad_table) prec = CUSOAP_PREC_F64;
+    return CUSOAP_OK;
+}
+
+// dL/dc (M, then W for a shared gradient, in ws) and the edge pass of centers
+// [0, n_centers) of the given arrays, adding into grad (not zeroed here)
+int vjp_core(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
+             const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
+             const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs,
+             int coeffs_dtype, const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast,
+             int n_feat, void* ws, double* grad, hipStream_t st) {
     const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED && !b->rad_table) ? 1 : 0;
     const int fold_k = (b->rbf == CUSOAP_RBF_GTO || b->rad_table) ? 1 : 0;   // the GTO / table edge kernels
     int rc;
@@ -869,7 +913,7 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
 #undef CUSOAP_ADJ
     if (rc) return rc;
 #define CUSOAP_CASE(L) \
-    case L: return launch_vjp_edges<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, grad, st);
+    case L: return vjp_edges_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, grad, st);
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
@@ -878,6 +922,24 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
 #undef CUSOAP_CASE
     return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: l_max out of range");
 }
+
+}  // namespace
+
+int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
+             const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
+             const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs, int coeffs_dtype,
+             const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast, int n_feat, void* ws,
+             size_t ws_bytes, double* grad, hipStream_t st) {
+    (void)n_img;
+    int rc = vjp_check(b, mode, prec, n_centers, center_species, pair_feature);
+    if (rc) return rc;
+    if (ws_bytes < vjp_ws_bytes(b, n_centers)) return fail(CUSOAP_ERR_WORKSPACE, "vjp: workspace too small");
+    if ((rc = zero_f64(grad, 3 * (int64_t)n_atoms, st))) return rc;
+    if (n_centers == 0) return CUSOAP_OK;
+    return vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
+                    offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad, st);
+}
+
 
 // ---------------------------------------------------------------------------
 // Loewdin S^-1/2 (float64, cyclic Jacobi; one thread per matrix)
@@ -942,5 +1004,6 @@ int lowdin_impl(const double* S, int batch, int n, double eps, double* out, hipS
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
 }
+#endif  // CUSOAP_ONLY_L
 
 }  // namespace cusoap

@@ -25,7 +25,15 @@ void prof_begin(const char* name, hipStream_t st) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
     hipEventRecord(e, st);
-    g_prof_open[name] = e;
+    // a begin whose end was skipped (an error return in between) leaves its event open:
+    // release it instead of leaking it
+    auto it = g_prof_open.find(name);
+    if (it != g_prof_open.end()) {
+        hipEventDestroy(it->second);
+        it->second = e;
+    } else {
+        g_prof_open[name] = e;
+    }
 }
 
 void prof_end(const char* name, hipStream_t st) {
@@ -104,17 +112,23 @@ int cusoap_profile_get(const char* name, double* total_ms, int32_t* count) {
     int n = 0;
     auto it = g_prof.find(name);
     if (it != g_prof.end()) {
-        for (auto& sp : it->second) {
+        // take the spans out of the map first, so that every event is destroyed exactly
+        // once whatever happens below
+        std::vector<ProfSpan> spans;
+        spans.swap(it->second);
+        g_prof.erase(it);
+        bool sync_ok = true;
+        for (auto& sp : spans) {
             float ms = 0.f;
-            if (hipEventSynchronize(sp.b) != hipSuccess) return fail(CUSOAP_ERR_HIP, "profile: event sync failed");
-            if (hipEventElapsedTime(&ms, sp.a, sp.b) == hipSuccess) {
+            if (sync_ok && hipEventSynchronize(sp.b) != hipSuccess) sync_ok = false;
+            if (sync_ok && hipEventElapsedTime(&ms, sp.a, sp.b) == hipSuccess) {
                 tot += ms;
                 ++n;
             }
             hipEventDestroy(sp.a);
             hipEventDestroy(sp.b);
         }
-        g_prof.erase(it);
+        if (!sync_ok) return fail(CUSOAP_ERR_HIP, "profile: event sync failed");
     }
     if (total_ms) *total_ms = tot;
     if (count) *count = n;

@@ -581,6 +581,26 @@ int dispatch_prec(const cusoap_basis_t& b, int prec, const double* pos, int n_at
 
 }  // namespace
 
+// build.py compiles this file once per l_max value with -DCUSOAP_ONLY_L=<l> (that object
+// instantiates the kernels of one degree, so the degrees compile in parallel) and once
+// without it, for the entry point below, which calls the per-degree functions.
+#define CUSOAP_COEFF_PARAMS const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, \
+    const int32_t* atom_sp, const double* shifts, const double* centers, int n_centers, const int32_t* self_sp, \
+    const int64_t* offsets, const int32_t* nbr, int create_path, void* out, hipStream_t st
+#define CUSOAP_COEFF_DECL(L) int coefficients_l##L(CUSOAP_COEFF_PARAMS);
+CUSOAP_COEFF_DECL(0) CUSOAP_COEFF_DECL(1) CUSOAP_COEFF_DECL(2) CUSOAP_COEFF_DECL(3) CUSOAP_COEFF_DECL(4)
+CUSOAP_COEFF_DECL(5) CUSOAP_COEFF_DECL(6) CUSOAP_COEFF_DECL(7) CUSOAP_COEFF_DECL(8) CUSOAP_COEFF_DECL(9)
+#undef CUSOAP_COEFF_DECL
+
+#ifdef CUSOAP_ONLY_L
+#define CUSOAP_COEFF_DEF(L)                                                                                    \
+    int coefficients_l##L(CUSOAP_COEFF_PARAMS) {                                                             \
+        return dispatch_prec<L>(b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, \
+                                create_path, out, st);                                                       \
+    }
+#define CUSOAP_COEFF_DEF2(L) CUSOAP_COEFF_DEF(L)
+CUSOAP_COEFF_DEF2(CUSOAP_ONLY_L)
+#else
 int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
                       const double* shifts, int n_img, const double* centers, int n_centers,
                       const int32_t* self_sp, const int64_t* offsets, const int32_t* nbr, int create_path,
@@ -596,7 +616,7 @@ int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int 
         return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: radial table larger than 2^31 values");
     if (n_centers == 0) return CUSOAP_OK;
 #define CUSOAP_CASE(L) \
-    case L: return dispatch_prec<L>(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st);
+    case L: return coefficients_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st);
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
@@ -605,5 +625,6 @@ int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int 
 #undef CUSOAP_CASE
     return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: l_max out of range");
 }
+#endif  // CUSOAP_ONLY_L
 
 }  // namespace cusoap

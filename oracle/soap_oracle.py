@@ -339,6 +339,16 @@ def radius_edges(centers: np.ndarray, neigh_pos: np.ndarray, cutoff: float,
     if C == 0 or M == 0:
         return np.zeros(0, np.int64), np.zeros(0, np.int64)
     cutoff2 = float(cutoff) * float(cutoff)
+    # Candidates outside the centers' bounding box grown by cutoff + 1e-6 cannot pass
+    # the test: their d^2 exceeds cutoff^2 by >= 2 cutoff 1e-6, far above the ~1e-11
+    # rounding of the formula at these coordinates.  Dropping them first leaves the
+    # same edges in the same order (np.nonzero keeps ascending candidate indices).
+    lo = centers.min(0) - (cutoff + 1e-6)
+    hi = centers.max(0) + (cutoff + 1e-6)
+    keep = np.nonzero(np.all((neigh_pos >= lo) & (neigh_pos <= hi), axis=1))[0]
+    if keep.size < M:
+        ci, ni = radius_edges(centers, neigh_pos[keep], cutoff, chunk)
+        return ci, keep[ni]
     nn = sqnorm3(neigh_pos)
     ci, ni = [], []
     for s in range(0, C, chunk):
