@@ -57,6 +57,8 @@ _SIGS = {
     "cusoap_vjp_workspace_size": [C.POINTER(Basis), i32, C.POINTER(sz)],
     "cusoap_vjp": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32, i32,
                    i32, p, sz, p, p],
+    "cusoap_vjp_noalloc": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32,
+                           i32, i32, p, p],
     "cusoap_lowdin_invsqrt_f64": [p, i32, i32, f64, p, p],
     "cusoap_radial_exact": [C.POINTER(Basis), p, i32, p, p, p],
     "cusoap_mlp_eval": [p, i32, i32, i32, p, i32, p, p, p],

@@ -28,6 +28,8 @@
 // float64 tile kernel.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "common.hpp"
@@ -860,8 +862,8 @@ size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) { return vjp_m_bytes
 
 namespace {
 
-// grad = 0 by a kernel rather than hipMemsetAsync (no memset node for callers that
-// capture HIP graphs)
+// grad = 0 by a kernel rather than hipMemsetAsync: the allocation-free entry point is
+// meant for callers that capture HIP graphs, where memset nodes are avoided
 __global__ void zero_f64_kernel(double* __restrict__ p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
@@ -923,6 +925,33 @@ int vjp_core(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
     return fail(CUSOAP_ERR_UNSUPPORTED, "vjp: l_max out of range");
 }
 
+// Workspace of the allocation-free entry point: a file-scope device array, one per GPU
+// (each device holds its own copy at its own address until the process ends).  In host
+// code `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed to kernels.
+constexpr size_t kNoAllocScratchBytes = CUSOAP_VJP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;      // device ordinal -> device address of `scratch`
+
+int scratch_address(void** out) {
+    int dev = 0;
+    CUSOAP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CUSOAP_ERR_HIP, std::string("vjp_noalloc: hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(dev, p).first;
+    }
+    *out = it->second;
+    return CUSOAP_OK;
+}
+
 }  // namespace
 
 int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
@@ -940,6 +969,42 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
                     offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad, st);
 }
 
+// Centers in chunks whose workspace fits the static array; the chunks add into grad.
+// Nothing in the array needs clearing: M and W are written before they are read.
+int vjp_noalloc_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms,
+                     const int32_t* atom_sp, const double* shifts, int n_img, const double* centers, int n_centers,
+                     const int32_t* center_attach, const int32_t* center_species, const int64_t* offsets,
+                     const int32_t* nbr, const void* coeffs, int coeffs_dtype, const int32_t* pair_feature,
+                     const void* gd, int gd_dtype, int gd_broadcast, int n_feat, double* grad, hipStream_t st) {
+    (void)n_img;
+    int rc = vjp_check(b, mode, prec, n_centers, center_species, pair_feature);
+    if (rc) return rc;
+    const size_t per_center = vjp_m_bytes(b, 1);     // one center's rows, rounded up to 256 bytes
+    const size_t wbytes = vjp_w_bytes(b);
+    if (per_center + wbytes > kNoAllocScratchBytes)
+        return fail(CUSOAP_ERR_CAPACITY, "vjp_noalloc: one center's adjoint rows exceed the 64 MiB scratch");
+    const int L2 = (b->l_max + 1) * (b->l_max + 1);
+    const size_t row_f64 = sizeof(double) * (size_t)b->n_species * b->n_max * L2;
+    int chunk = (int)std::min<size_t>((kNoAllocScratchBytes - wbytes - 256) / row_f64, (size_t)INT32_MAX);
+    while (chunk > 1 && vjp_ws_bytes(b, chunk) > kNoAllocScratchBytes) --chunk;
+    if (chunk < 1 || vjp_ws_bytes(b, chunk) > kNoAllocScratchBytes)
+        return fail(CUSOAP_ERR_CAPACITY, "vjp_noalloc: basis too large for the 64 MiB scratch");
+    void* ws = nullptr;
+    if ((rc = scratch_address(&ws))) return rc;
+    if ((rc = zero_f64(grad, 3 * (int64_t)n_atoms, st))) return rc;
+    const size_t csz = coeffs_dtype == CUSOAP_DT_F64 ? 8 : 4, gsz = gd_dtype == CUSOAP_DT_F64 ? 8 : 4;
+    const size_t per = (size_t)b->n_species * b->n_max * L2;
+    for (int c0 = 0; c0 < n_centers; c0 += chunk) {
+        const int nc = std::min(chunk, n_centers - c0);
+        rc = vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers + 3 * (size_t)c0, nc,
+                      center_attach ? center_attach + c0 : nullptr, center_species ? center_species + c0 : nullptr,
+                      offsets + c0, nbr, (const char*)coeffs + csz * per * c0, coeffs_dtype, pair_feature,
+                      gd_broadcast ? gd : (const void*)((const char*)gd + gsz * (size_t)n_feat * c0), gd_dtype,
+                      gd_broadcast, n_feat, ws, grad, st);
+        if (rc) return rc;
+    }
+    return CUSOAP_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Loewdin S^-1/2 (float64, cyclic Jacobi; one thread per matrix)

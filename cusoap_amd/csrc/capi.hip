@@ -74,6 +74,9 @@ size_t vjp_ws_bytes(const cusoap_basis_t*, int);
 int vjp_impl(const cusoap_basis_t*, int, int, const double*, int, const int32_t*, const double*, int, const double*,
              int, const int32_t*, const int32_t*, const int64_t*, const int32_t*, const void*, int, const int32_t*,
              const void*, int, int, int, void*, size_t, double*, hipStream_t);
+int vjp_noalloc_impl(const cusoap_basis_t*, int, int, const double*, int, const int32_t*, const double*, int,
+                     const double*, int, const int32_t*, const int32_t*, const int64_t*, const int32_t*, const void*,
+                     int, const int32_t*, const void*, int, int, int, double*, hipStream_t);
 int lowdin_impl(const double*, int, int, double, double*, hipStream_t);
 int64_t mlp_param_count(int, int);
 int radial_exact_impl(const cusoap_basis_t*, const double*, int, double*, double*, hipStream_t);
@@ -207,6 +210,17 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision, con
     return vjp_impl(basis, mode, precision, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
                     center_attach, center_species, offsets, nbr, coeffs, coeffs_dtype, pair_feature, grad_desc,
                     gd_dtype, gd_broadcast, n_feat, ws, ws_bytes, grad_pos, (hipStream_t)stream);
+}
+
+int cusoap_vjp_noalloc(const cusoap_basis_t* basis, int32_t mode, int32_t precision, const double* positions,
+                       int32_t n_atoms, const int32_t* atom_species, const double* shifts, int32_t n_img,
+                       const double* centers, int32_t n_centers, const int32_t* center_attach,
+                       const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs,
+                       int32_t coeffs_dtype, const int32_t* pair_feature, const void* grad_desc, int32_t gd_dtype,
+                       int32_t gd_broadcast, int32_t n_feat, double* grad_pos, void* stream) {
+    return vjp_noalloc_impl(basis, mode, precision, positions, n_atoms, atom_species, shifts, n_img, centers,
+                            n_centers, center_attach, center_species, offsets, nbr, coeffs, coeffs_dtype,
+                            pair_feature, grad_desc, gd_dtype, gd_broadcast, n_feat, grad_pos, (hipStream_t)stream);
 }
 
 int cusoap_lowdin_invsqrt_f64(const double* S, int32_t batch, int32_t n, double eps, double* out, void* stream) {

@@ -222,6 +222,26 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
                const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
                void* ws, size_t ws_bytes, double* grad_pos, void* stream);
 
+/* Allocation-free cusoap_vjp.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving or MD loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as cusoap_vjp without the
+ * workspace: the dL/dc rows live in a static device array of
+ * CUSOAP_VJP_NOALLOC_SCRATCH_BYTES (64 MiB) that every GPU the library runs on holds until
+ * the process ends, and the centers are processed in as many chunks as that array needs
+ * (forces add up over the chunks).  Calls no hipMalloc, hipFree or allocating library
+ * routine, and zeroes grad_pos with a kernel (no memset node).  All calls on one GPU share
+ * its array: calls that overlap in time on one GPU (other streams, other threads) race.
+ * Returns CUSOAP_ERR_CAPACITY when a single center's rows do not fit the array. */
+#define CUSOAP_VJP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int cusoap_vjp_noalloc(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
+                       const double* positions, int32_t n_atoms, const int32_t* atom_species,
+                       const double* shifts, int32_t n_img,
+                       const double* centers, int32_t n_centers, const int32_t* center_attach,
+                       const int32_t* center_species, const int64_t* offsets, const int32_t* nbr,
+                       const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
+                       const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
+                       double* grad_pos, void* stream);
+
 /* ---- Loewdin orthonormalisation S^-1/2 in float64 (replaces
  *      _lowdin_invsqrt, soap_generator.py:339-343, which calls torch.linalg.eigh
  *      on the device) ----------------------------------------------------------
