@@ -206,6 +206,12 @@ class SOAP:
             raise ValueError("n_max and l_max must be provided.")
         if species is None or len(species) == 0:
             raise ValueError("species must be provided (list of atomic numbers or symbols).")
+        # constructor arguments, for objects of the same basis in another form (_sibling)
+        self._init_kwargs = dict(r_cut=r_cut, n_max=n_max, l_max=l_max, sigma=sigma, rbf=rbf, weighting=weighting,
+                                 crossover=crossover, average=average, species=list(species), periodic=periodic,
+                                 sparse=sparse, dtype=dtype, device=device, quad_n=quad_n,
+                                 max_num_neighbors=max_num_neighbors, precision=precision,
+                                 table_intervals=table_intervals, spectrum=spectrum, _betas=_betas)
         self._dtype_str = dtype
         self.dtype = _torch_dtype_from_str(dtype)
         self.device = torch.device(device) if device is not None else \
@@ -286,9 +292,15 @@ class SOAP:
             self._rad_intervals = int(table_intervals)
             self._rad_table = radial_table(self, self._rad_intervals)
 
+    def _sibling(self, **overrides) -> "SOAP":
+        """A SOAP of the same basis (same Loewdin matrices) with some arguments replaced."""
+        kw = dict(self._init_kwargs, _betas=self._betas)
+        kw.update(overrides)
+        return SOAP(**kw)
+
     def _init_learned_radial(self, surrogate, n_intervals: int):
         """Train or load the radial networks, gate them, tabulate the radial model."""
-        from .surrogate import RadialSurrogate, radial_table
+        from .surrogate import RadialSurrogate, masked_table, radial_table
         self._require_gpu()
         if self._rbf != "gto":
             raise NotImplementedError("radial='learned' covers the GTO basis")
@@ -298,15 +310,22 @@ class SOAP:
             opts = dict(surrogate) if isinstance(surrogate, dict) else {}
             ckpt = surrogate if isinstance(surrogate, str) else opts.pop("checkpoint", None)
             init_keys = ("r_fit", "n_train", "n_heldout", "seed", "deriv_weight", "tol", "tol_slope")
+            gate_keys = ("calibration", "bar", "margin")
+            gate_opts = {k: opts.pop(k) for k in gate_keys if k in opts}
             model = RadialSurrogate(self, **{k: opts.pop(k) for k in init_keys if k in opts})
             if ckpt:
                 model.load(ckpt)
             else:
                 model.train(**opts)
-        model.gate()
-        self.surrogate = model
+            model.gate_options = gate_opts
         self._rad_intervals = n_intervals
-        self._rad_table = radial_table(self, n_intervals, model=model)
+        self.surrogate = model
+        channels = model.gate(self, n_intervals)
+        self._learned_channels = channels
+        self._learned_use_pair = channels.reshape(channels.shape[0], -1).any(1).astype(np.int32)
+        exact = radial_table(self, n_intervals)
+        net = radial_table(self, n_intervals, model=model, use_pair=self._learned_use_pair)
+        self._rad_table = masked_table(exact, net, channels)
 
     # ---- basis (soap_generator.py:690-819) ---------------------------------
     def _init_gto_basis(self, betas=None):

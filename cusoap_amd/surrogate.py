@@ -217,11 +217,112 @@ class RadialSurrogate:
         self.errors = {"value": ev.cpu().numpy(), "slope": ed.cpu().numpy()}
         return self.errors
 
-    def gate(self) -> np.ndarray:
-        """Enable the network of every pair whose held-out errors are within the bounds."""
+    def gate(self, soap=None, n_intervals: int = 1024) -> np.ndarray:
+        """Channels (species pair, l, n) whose network the radial table of ``soap`` may use.
+
+        Returns a boolean mask (S*S, l_max+1, n_max).  Two stages, the exact function
+        judging both:
+          1. fit: the channels of a pair pass when the pair's held-out errors are within
+             tol (values) and tol_slope (slopes), relative to each function's maximum;
+          2. end to end: on a small calibration system of the basis' species (a periodic
+             water box for H/O, else a random box; ``gate_options["calibration"]``
+             replaces it) the descriptors and the forces of a seeded Gaussian readout
+             from the learned table are compared with those from the exact table (same
+             kernels, float64 arithmetic).  Each channel's own contribution to those errors
+             is measured once (table with that channel alone on its network); the errors
+             of a set of channels are, to first order, the sums of these contributions.
+             While the predicted larger error exceeds margin * bar (default 0.35 * 1e-4:
+             the largest errors of other systems of the same kind were measured at up to
+             ~2.4x those of the calibration system),
+             the channel whose return to the exact function lowers it most goes back;
+             the chosen set is then evaluated for real and trimmed further if needed.
+        """
+        soap = self.soap if soap is None else soap
+        opts = getattr(self, "gate_options", {}) or {}
+        bar, margin = float(opts.get("bar", 1e-4)), float(opts.get("margin", 0.35))
+        bound = margin * bar
         e = self.heldout_errors()
-        self.use_pair = ((e["value"] <= self.tol) & (e["slope"] <= self.tol_slope)).astype(np.int32)
-        return self.use_pair
+        fit = (e["value"] <= self.tol) & (e["slope"] <= self.tol_slope)
+        mask = np.repeat(fit[:, None, None], self.L1 * self.n).reshape(self.n_pairs, self.L1, self.n)
+        rep = {"radial_heldout_value_err": float(e["value"].max()),
+               "radial_heldout_slope_err": float(e["slope"].max()),
+               "radial_fit_gate": [int(u) for u in fit], "radial_gate_bound": bound}
+        if mask.any():
+            calib = opts.get("calibration") or calibration_system(soap)
+            # float64 arithmetic: the deviations are the network's alone, and repeat bit for
+            # bit closely enough that the selection does not depend on float32 rounding
+            ref = soap._sibling(radial="table", table_intervals=n_intervals, dtype="float64",
+                                precision="float64", average=soap.average)
+            exact_tab = ref._rad_table
+            net_tab = radial_table(soap, n_intervals, model=self, use_pair=fit.astype(np.int32))
+            w = torch.as_tensor(np.random.default_rng(777).normal(size=ref.get_number_of_features()),
+                                device=soap.device)
+            d_ref, F_ref = ref.descriptor_and_vjp(calib, w)
+            d_ref = d_ref.double()
+            d_sc = d_ref.abs().amax(0).clamp_min(1e-300)
+            F_sc = F_ref.abs().max().clamp_min(1e-300)
+
+            def deviation(m):
+                """Normalised descriptor and force deviations of the table with channels m."""
+                ref._rad_table = masked_table(exact_tab, net_tab, m)
+                ref._basis_learned = None
+                d, F = ref.descriptor_and_vjp(calib, w)
+                return (d.double() - d_ref) / d_sc, ((F - F_ref) / F_sc).reshape(-1)
+
+            def norms(dv, fv):
+                return float(dv.abs().max()), float(fv.abs().max())
+
+            err = norms(*deviation(mask))
+            rep["radial_calibration_err_all_fit_channels"] = {"desc": err[0], "force": err[1]}
+            chans = [tuple(c) for c in np.argwhere(mask)]
+            if max(err) > bound:
+                # one deviation per channel; the set's deviation is their sum (first order);
+                # descriptor rows of at most 256 centers enter the model, all rows the checks
+                rows = min(256, int(d_ref.shape[0]))
+                Ed, Ef = [], []
+                for c in chans:
+                    one = np.zeros_like(mask)
+                    one[c] = True
+                    dv, fv = deviation(one)
+                    Ed.append(dv[:rows].reshape(-1).float())     # the first rows stand for all
+                    Ef.append(fv.float())
+                Ed, Ef = torch.stack(Ed), torch.stack(Ef)
+                keep = torch.ones(len(chans), dtype=torch.bool, device=soap.device)
+                Sd, Sf = Ed.sum(0), Ef.sum(0)
+                while bool(keep.any()):
+                    cur = max(float(Sd.abs().max()), float(Sf.abs().max()))
+                    if cur <= bound:
+                        m = np.zeros_like(mask)
+                        for k in torch.nonzero(keep).flatten().tolist():
+                            m[chans[k]] = True
+                        err = norms(*deviation(m))
+                        if max(err) <= bound:
+                            break
+                    # the kept channel whose removal leaves the smallest larger error
+                    cand = torch.nonzero(keep).flatten()
+                    after = torch.maximum((Sd[None] - Ed[cand]).abs().amax(1), (Sf[None] - Ef[cand]).abs().amax(1))
+                    k = int(cand[int(torch.argmin(after))])
+                    keep[k] = False
+                    Sd, Sf = Sd - Ed[k], Sf - Ef[k]
+                mask = np.zeros_like(mask)
+                for k in torch.nonzero(keep).flatten().tolist():
+                    mask[chans[k]] = True
+                if not mask.any():
+                    err = (0.0, 0.0)
+                del Ed, Ef
+            rep["radial_calibration_err"] = {"desc": err[0], "force": err[1]}
+            rep["radial_calibration_atoms"] = int(len(calib["positions"]))
+            ref._rad_table, ref._basis_learned = exact_tab, None
+        rep["radial_channels_learned"] = int(mask.sum())
+        rep["radial_channels_total"] = int(mask.size)
+        self.channels = mask
+        self.use_pair = mask.reshape(self.n_pairs, -1).any(1).astype(np.int32)
+        self.report = rep
+        return mask.copy()
+
+    def gate_report(self) -> dict:
+        """What the last gate measured: held-out errors, the fit gate, calibration errors."""
+        return dict(getattr(self, "report", {}))
 
     # ---- tabulation -------------------------------------------------------------
     def table(self, n_intervals: int = 1024, r_max: Optional[float] = None, exact: bool = False) -> torch.Tensor:
@@ -231,7 +332,8 @@ class RadialSurrogate:
     # ---- checkpoint ---------------------------------------------------------------
     def _meta(self):
         s = self.soap
-        return np.array([s._r_cut, s._sigma, s._n_max, s._l_max, self.S, self.r_fit, HIDDEN], dtype=np.float64)
+        return np.array([s._r_cut, s._sigma, s._n_max, s._l_max, self.S, self.r_fit, HIDDEN, self.seed,
+                         self.n_train], dtype=np.float64)
 
     def save(self, path: str):
         np.savez(path, params=self.params.cpu().numpy(), meta=self._meta(), species=np.array(self.soap._species_Z),
@@ -244,9 +346,42 @@ class RadialSurrogate:
             raise ValueError("checkpoint was trained for a different basis")
         if np.abs(d["betas"] - np.asarray(self.soap._betas)).max() > 1e-9 * np.abs(d["betas"]).max():
             raise ValueError("checkpoint was trained for a different orthonormalisation")
+        # the output scales are recomputed from the same seeded samples in __init__; a
+        # checkpoint whose saved scales differ would rescale the network outputs
+        for key, mine in (("scale", self.scale), ("dscale", self.dscale)):
+            saved = torch.as_tensor(d[key], device=self.device, dtype=torch.float64)
+            if saved.shape != mine.shape or (saved - mine).abs().max() > 1e-12 * mine.abs().max():
+                raise ValueError(f"checkpoint {key} does not match this basis and training set")
+            mine.copy_(saved)
         self.params = torch.from_numpy(d["params"].astype(np.float32)).to(self.device).contiguous()
         self.step = int(d["step"])
         return self
+
+
+def masked_table(exact: torch.Tensor, net: torch.Tensor, channels: np.ndarray) -> torch.Tensor:
+    """Radial table with the network's nodes on the given (pair, l, n) channels and the
+    exact nodes everywhere else (slots of coordinate centers stay exact)."""
+    n_pairs = channels.shape[0]
+    full = torch.zeros(exact.shape[0], *channels.shape[1:], dtype=torch.bool, device=exact.device)
+    full[:n_pairs] = torch.as_tensor(channels, device=exact.device)
+    return torch.where(full[:, None, :, :, None], net, exact).contiguous()
+
+
+def calibration_system(soap, seed: int = 0) -> dict:
+    """Small system of the basis' species for the end-to-end gate: a periodic 8^3 water box
+    (1,536 atoms, 1.0 g/cm^3) for H/O, else a random periodic box of 1,536 atoms at
+    0.10 A^-3 (H/C/N/O in the fractions 0.50/0.25/0.10/0.15, other species in equal parts).
+    Without periodic=True the same atoms are taken as a cluster."""
+    from . import systems
+    Z = list(soap._species_Z)
+    if Z == [1, 8]:
+        s = systems.water_box(8, seed=seed)
+    else:
+        fr = (0.50, 0.25, 0.10, 0.15) if Z == [1, 6, 7, 8] else tuple([1.0 / len(Z)] * len(Z))
+        s = systems.random_box(n_atoms=1536, fractions=fr, Z=Z, density=0.10, seed=seed)
+    if not soap.periodic:
+        s = {"positions": s["positions"], "atomic_numbers": s["atomic_numbers"]}
+    return s
 
 
 def exact_radial(soap, r: np.ndarray):
@@ -262,10 +397,11 @@ def exact_radial(soap, r: np.ndarray):
 
 
 def radial_table(soap, n_intervals: int = 1024, r_max: Optional[float] = None,
-                 model: Optional[RadialSurrogate] = None) -> torch.Tensor:
+                 model: Optional[RadialSurrogate] = None, use_pair: Optional[np.ndarray] = None) -> torch.Tensor:
     """(S+1) S x (n_intervals+1) x (l_max+1) x n_max x 2 float32 table on the device.
 
-    The last axis holds (B_nl(r_i), dB_nl/dr(r_i)) at node r_i."""
+    The last axis holds (B_nl(r_i), dB_nl/dr(r_i)) at node r_i.  With a model, the nodes
+    of the pairs in ``use_pair`` (default: the model's last gate) come from the networks."""
     lib = _lib.load()
     b = soap._basis_exact()
     r_max = float(soap._cutoff if r_max is None else r_max)
@@ -279,7 +415,8 @@ def radial_table(soap, n_intervals: int = 1024, r_max: Optional[float] = None,
     _lib.check(lib.cusoap_radial_table_workspace_size(C.byref(b), n_intervals, C.byref(nbytes)),
                "radial_table_workspace_size")
     ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=soap.device)
-    use = torch.as_tensor(model.use_pair.astype(np.int32), device=soap.device)
+    use = torch.as_tensor(np.asarray(model.use_pair if use_pair is None else use_pair, dtype=np.int32),
+                          device=soap.device)
     scale = model.scale.float().expand(model.n_pairs, model.O).contiguous()
     _lib.check(lib.cusoap_radial_table(C.byref(b), model.params.data_ptr(), HIDDEN, use.data_ptr(),
                                        scale.data_ptr(), model.r_fit, n_intervals, r_max, ws.data_ptr(), ws.numel(),

@@ -4,12 +4,13 @@
 
 Tolerances: float32 network kernels vs the float64 restatement at 1e-5 of the
 output scale (2e-4 for gradients, which sum float32 products over a tile).
-The trained networks must pass their gate: held-out error of g_nl <= 2e-5 and of
-its slope <= 1e-4, relative to each function's maximum.  Learned descriptors are
-held to 2e-4 and forces to 5e-4 relative (max over feature columns of
-max|d - ref| / max|ref|; forces max|F - ref| / max|ref|).  These are the levels
-the 64-64 networks reach; the plain float32 method misses 1e-4 on descriptors by
-8x, and the test requires the learned form to beat it there.
+The trained networks must pass the fit stage of their gate: held-out error of g_nl
+<= 2e-5 and of its slope <= 1e-4, relative to each function's maximum.  Learned
+descriptors and forces are held to 1e-4 relative against the float64 oracle (max
+over feature columns of max|d - ref| / max|ref|; forces max|F - ref| / max|ref|):
+the end-to-end stage of the gate returns pairs to the exact function until a
+calibration system meets half of that.  The plain float32 method misses 1e-4 on
+descriptors by 8x, and the test requires the learned form to beat it there.
 """
 
 import ctypes as C
@@ -181,10 +182,16 @@ def learned():
 
 
 def test_training_meets_the_gate(learned):
+    """Every pair passes the fit stage; the end-to-end stage leaves the calibration errors
+    within its bound (0.35e-4) with channels still on the networks."""
     o, g = learned
     e = g.surrogate.errors
-    assert np.all(g.surrogate.use_pair == 1)
+    rep = g.surrogate.gate_report()
+    say("gate report:", rep)
     assert e["value"].max() <= 2e-5 and e["slope"].max() <= 1e-4
+    assert all(u == 1 for u in rep["radial_fit_gate"])
+    assert rep["radial_calibration_err"]["desc"] <= 0.35e-4 and rep["radial_calibration_err"]["force"] <= 0.35e-4
+    assert g.surrogate.use_pair.any()
 
 
 @pytest.mark.parametrize("n_max,l_max", [(8, 6), (7, 3), (6, 5), (5, 4), (4, 2), (3, 4)])
@@ -224,9 +231,10 @@ def test_learned_descriptor_and_forces_within_bound(learned):
     for name, soap in (("learned", gl), ("float32", g32)):
         d, F = soap.descriptor_and_vjp(s, torch.tensor(w))
         errs[name] = (colerr(d, ref), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max())
-    say("desc/force errors vs float64 oracle:", errs)
-    assert errs["learned"][0] <= 2e-4 and errs["learned"][1] <= 5e-4
+    say("desc/force errors vs float64 oracle:", errs, "pairs learned:", gl._learned_use_pair)
+    assert errs["learned"][0] <= 1e-4 and errs["learned"][1] <= 1e-4
     assert errs["learned"][0] < errs["float32"][0]
+    assert gl._learned_use_pair.any()
 
 
 def test_checkpoint_roundtrip(learned, tmp_path):
@@ -238,3 +246,47 @@ def test_checkpoint_roundtrip(learned, tmp_path):
     assert torch.equal(g2._rad_table, g._rad_table)
     s = systems.water_cluster_300(seed=3)
     assert torch.equal(g2.create(s), g.create(s))
+
+
+HCNO = dict(r_cut=5.0, n_max=8, l_max=6, sigma=0.4, species=[1, 6, 7, 8], periodic=True)
+
+
+def test_exact_table_multi_species_and_coordinate_centers():
+    """Four species (the 4-species accumulation and the edge pass's 5-species groups) and
+    centers that mix atom indices with coordinates (the table's coordinate-center slot)."""
+    s = systems.random_box(n_atoms=400, density=0.1, seed=3)
+    o = so.OracleSOAP(**HCNO)
+    w = np.random.default_rng(8).normal(size=o.get_number_of_features())
+    Fr = o.vjp(s, w[None, :].repeat(len(s["positions"]), 0))
+    ref = o.create(s)
+    cen = [0, 17, [1.0, 2.0, 3.0], 399, [10.25, 0.5, 7.75], 5]
+    ref_c = o.create(s, centers=cen)
+    for prec, tol in (("float64", (1e-6, 2e-5)), ("mixed", (1e-5, 2e-5))):
+        g = SOAP(device=DEV, dtype="float64", precision=prec, radial="table", _betas=o.betas, **HCNO)
+        d, F = g.descriptor_and_vjp(s, torch.tensor(w))
+        ed, ef = colerr(d, ref), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
+        ec = colerr(g.create(s, centers=cen), ref_c)
+        say(f"exact table H/C/N/O ({prec}): desc {ed:.2e}, forces {ef:.2e}, mixed centers {ec:.2e}")
+        assert ed < tol[0] and ef < tol[1] and ec < tol[0]
+
+
+def test_learned_hcno_sixteen_pairs():
+    """The H/C/N/O basis (r_cut 5, sigma 0.4, n 8, l 6): 16 pair networks trained on the
+    device, gated, against the float64 oracle on a 400-atom periodic box."""
+    o = so.OracleSOAP(**HCNO)
+    say("training 16 radial networks (H/C/N/O, n8 l6)")
+    g = SOAP(device=DEV, dtype="float64", precision="mixed", radial="learned", _betas=o.betas, **HCNO)
+    rep = g.surrogate.gate_report()
+    say("  trained in", round(g.surrogate.train_seconds, 1), "s; gate", rep)
+    s = systems.random_box(n_atoms=400, density=0.1, seed=9)
+    w = np.random.default_rng(10).normal(size=o.get_number_of_features())
+    ref = o.create(s)
+    Fr = o.vjp(s, w[None, :].repeat(len(s["positions"]), 0))
+    d, F = g.descriptor_and_vjp(s, torch.tensor(w))
+    ed, ef = colerr(d, ref), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
+    g32 = SOAP(device=DEV, dtype="float64", precision="float32", _betas=o.betas, **HCNO)
+    ed32 = colerr(g32.create(s), ref)
+    say(f"learned H/C/N/O: desc {ed:.2e} (plain float32 {ed32:.2e}), forces {ef:.2e}, "
+        f"channels learned {rep['radial_channels_learned']} of {rep['radial_channels_total']}")
+    assert ed <= 1e-4 and ef <= 1e-4 and ed < ed32
+    assert rep["radial_channels_learned"] > 0

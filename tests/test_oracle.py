@@ -50,6 +50,27 @@ def test_sh_closed_forms_l3():
     np.testing.assert_allclose(so.spherical_harmonics(3, u), _table_l3(u), rtol=0, atol=1e-14)
 
 
+def test_sh_against_scipy_complex_harmonics_l9():
+    """Sign and m-order for every degree up to 9, from scipy's complex Y_l^m (which carries
+    the Condon-Shortley phase): the real basis without that phase is
+    Y_l0 = Y_l^0, Y_lm = sqrt(2) (-1)^m Re Y_l^m and Y_l,-m = sqrt(2) (-1)^m Im Y_l^m (m > 0),
+    so Y_1,-1..1 are proportional to +y, +z, +x as the closed forms above state.  A swapped
+    sign, a swapped cos/sin order or an m-reversed block fails here, unlike orthonormality
+    or the addition theorem."""
+    u = _unit(np.random.default_rng(11), 300)
+    theta = np.arccos(np.clip(u[:, 2], -1.0, 1.0))
+    phi = np.arctan2(u[:, 1], u[:, 0])
+    lmax = 9
+    ref = np.zeros((u.shape[0], (lmax + 1) ** 2))
+    for l in range(lmax + 1):
+        ref[:, l * l + l] = special.sph_harm_y(l, 0, theta, phi).real
+        for m in range(1, l + 1):
+            ylm = special.sph_harm_y(l, m, theta, phi)
+            ref[:, l * l + l + m] = math.sqrt(2.0) * (-1) ** m * ylm.real
+            ref[:, l * l + l - m] = math.sqrt(2.0) * (-1) ** m * ylm.imag
+    np.testing.assert_allclose(so.spherical_harmonics(lmax, u), ref, rtol=0, atol=1e-12)
+
+
 def test_solid_harmonics_homogeneous():
     rng = np.random.default_rng(2)
     u = _unit(rng, 50)
