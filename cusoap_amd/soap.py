@@ -562,12 +562,17 @@ class SOAP:
                         idx[k] = j
         return xyz, idx
 
-    def _context(self, system, centers) -> _Ctx:
+    def _context(self, system, centers, mark=None) -> _Ctx:
+        """Device-resident inputs; ``mark(stage)`` (optional) is called after the input
+        conversion ("to_tensors") and after the centers ("centers")."""
         self._require_gpu()
         pos, Z, cell, pbc = system_to_arrays(system, keep_device=True)
         if centers is None and self._species_cache is not None:
             x = self._context_cached(pos, Z, cell, pbc)
             if x is not None:
+                if mark:
+                    mark("to_tensors")
+                    mark("centers")
                 return x
         bad = set(np.unique(Z).tolist()) - self._atomic_number_set
         if bad:
@@ -579,6 +584,8 @@ class SOAP:
         else:
             x.pos = self._dev(pos)
             x.n_atoms = pos.shape[0]
+        if mark:
+            mark("to_tensors")
         if centers is None:
             x.centers = x.pos
             cidx = np.arange(x.n_atoms, dtype=np.int64)
@@ -586,6 +593,8 @@ class SOAP:
             pos_h = pos.cpu().numpy() if isinstance(pos, torch.Tensor) else pos
             cxyz, cidx = self.prepare_centers(pos_h, centers)
             x.centers = self._dev(np.ascontiguousarray(cxyz))
+        if mark:
+            mark("centers")
         if self.periodic:
             if cell is None or pbc is None:
                 raise ValueError("periodic=True requires cell and pbc.")
@@ -630,23 +639,24 @@ class SOAP:
         return x
 
     # ---- pipeline stages ----------------------------------------------------------
-    def _neighbors(self, x: _Ctx, drop_self: bool) -> _NL:
+    def _neighbors(self, x: _Ctx, drop_self: bool, cutoff: Optional[float] = None) -> _NL:
         lib = _lib.load()
         st = self._stream()
+        cutoff = self._cutoff if cutoff is None else float(cutoff)
         plan = _lib.NLPlan()
         tmp = torch.empty(16384, dtype=torch.uint8, device=self.device)
-        _lib.check(lib.cusoap_nl_plan(x.centers.data_ptr(), x.n_centers, x.n_atoms, x.n_img, self._cutoff,
+        _lib.check(lib.cusoap_nl_plan(x.centers.data_ptr(), x.n_centers, x.n_atoms, x.n_img, cutoff,
                                       tmp.data_ptr(), C.byref(plan), st), "nl_plan")
         ws = torch.empty(int(plan.ws_bytes), dtype=torch.uint8, device=self.device)
         offsets = torch.empty(x.n_centers + 1, dtype=torch.int64, device=self.device)
         total, mx = C.c_int64(0), C.c_int32(0)
         _lib.check(lib.cusoap_nl_count(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
-                                       x.centers.data_ptr(), x.n_centers, self._cutoff, int(drop_self),
+                                       x.centers.data_ptr(), x.n_centers, cutoff, int(drop_self),
                                        ws.data_ptr(), ws.numel(), offsets.data_ptr(), C.byref(total), C.byref(mx),
                                        st), "nl_count")
         nbr = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=self.device)
         _lib.check(lib.cusoap_nl_fill(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
-                                      x.centers.data_ptr(), x.n_centers, self._cutoff, int(drop_self),
+                                      x.centers.data_ptr(), x.n_centers, cutoff, int(drop_self),
                                       ws.data_ptr(), ws.numel(), offsets.data_ptr(), mx.value, nbr.data_ptr(), st),
                    "nl_fill")
         return _NL(offsets, nbr, int(total.value), int(mx.value))
@@ -669,22 +679,24 @@ class SOAP:
                    "coefficients")
         return out
 
-    def _descriptor(self, coeffs: torch.Tensor, average: Optional[str] = None) -> torch.Tensor:
+    def _descriptor(self, coeffs: torch.Tensor, average: Optional[str] = None, dtype: Optional[torch.dtype] = None,
+                    exact_spectrum: bool = False) -> torch.Tensor:
         avg = self.average if average is None else average
+        dtype = self.dtype if dtype is None else dtype
         lib = _lib.load()
         b = self._basis()
         cdt = _lib.DT_F64 if coeffs.dtype == torch.float64 else _lib.DT_F32
-        odt = _lib.DT_F64 if self.dtype == torch.float64 else _lib.DT_F32
+        odt = _lib.DT_F64 if dtype == torch.float64 else _lib.DT_F32
         C_ = coeffs.shape[0]
         if avg == "cc":
-            out = torch.empty((C_, self.n_species * self._n_max * (self._l_max + 1) ** 2), dtype=self.dtype,
+            out = torch.empty((C_, self.n_species * self._n_max * (self._l_max + 1) ** 2), dtype=dtype,
                               device=self.device)
             _lib.check(lib.cusoap_cc_projection(C.byref(b), coeffs.data_ptr(), cdt, C_, out.data_ptr(), odt,
                                                 self._stream()), "cc_projection")
             return out
         nf = self._n_power_features()
-        out = torch.empty((C_, nf), dtype=self.dtype, device=self.device)
-        if self.spectrum != "exact":
+        out = torch.empty((C_, nf), dtype=dtype, device=self.device)
+        if self.spectrum != "exact" and not exact_spectrum:
             kind, split = _SPECTRA[self.spectrum]
             _lib.check(lib.cusoap_power_spectrum_mfma(C.byref(b), coeffs.data_ptr(), cdt, C_,
                                                       self._keep["pf"].data_ptr(), nf, kind, split, out.data_ptr(),
@@ -738,30 +750,59 @@ class SOAP:
         nl = self._neighbors(x, drop_self=drop_self)
         return nl.offsets, nl.nbr[:nl.n_edges]
 
+    # stages of the reference's coefficient step (soap_generator.py:1328-1391, 821-850) that
+    # one fused kernel performs here
+    _FUSED_STAGES = ("ylm", "weights", "radial", "scatter", "beta", "self_terms")
+
     @torch.no_grad()
     def create_with_profile(self, system, centers=None):
-        """Descriptor, per-stage seconds and run statistics (soap_generator.py:1815-1868)."""
+        """Descriptor, per-stage seconds and run statistics (soap_generator.py:1815-1868).
+
+        The keys are the reference's: to_tensors, centers, neighbor, ylm, weights, radial,
+        scatter, beta, self_terms, power_spectrum, total.  to_tensors, centers and neighbor
+        (which synchronises on its edge count) are wall times with the stream synchronised
+        at each end, as the reference's Profiler measures them; the device stages are the
+        HIP-event times of their kernels on the launch stream.  The reference's ylm ...
+        self_terms run inside one kernel here: its time is reported under "coefficients",
+        and those keys hold 0.0 (``extra["fused_into_coefficients"]`` lists them).
+        """
         times: Dict[str, float] = {}
+        clock = [0.0]
 
-        def mark(name, t0):
+        def sync():
             torch.cuda.synchronize(self.device)
-            times[name] = time.perf_counter() - t0
-            return time.perf_counter()
 
-        torch.cuda.synchronize(self.device)
-        t_all = t = time.perf_counter()
-        x = self._context(system, centers)
-        t = mark("to_tensors", t)
-        nl = self._neighbors(x, drop_self=True)
-        t = mark("neighbor", t)
-        coeffs = self._coefficients(x, nl, create_path=True)
-        t = mark("coefficients", t)
-        out = self._descriptor(coeffs)
-        t = mark("power_spectrum", t)
-        times["total"] = time.perf_counter() - t_all
+        def mark(name):
+            sync()
+            now = time.perf_counter()
+            times[name] = times.get(name, 0.0) + (now - clock[0])
+            clock[0] = now
+
+        kernels = ("nl_count", "nl_fill", "coefficients", "power_spectrum", "cc_projection")
+        _lib.profile_enable(True)
+        try:
+            for k in kernels:
+                _lib.profile_get(k)
+            sync()
+            t_all = clock[0] = time.perf_counter()
+            x = self._context(system, centers, mark=mark)
+            nl = self._neighbors(x, drop_self=True)
+            mark("neighbor")
+            coeffs = self._coefficients(x, nl, create_path=True)
+            out = self._descriptor(coeffs)
+            sync()
+            times["total"] = time.perf_counter() - t_all
+            kms = {k: _lib.profile_get(k)[0] for k in kernels}
+        finally:
+            _lib.profile_enable(False)
+        for k in self._FUSED_STAGES:
+            times[k] = 0.0
+        times["coefficients"] = kms["coefficients"] * 1e-3
+        times["power_spectrum"] = (kms["power_spectrum"] + kms["cc_projection"]) * 1e-3
         extra = {"n_atoms": x.n_atoms, "n_centers": x.n_centers, "E_edges": nl.n_edges,
                  "neighbors_per_center": nl.n_edges / x.n_centers if x.n_centers else 0.0,
-                 "nl_backend": self._last_nl_backend}
+                 "nl_backend": self._last_nl_backend, "kernel_ms": kms,
+                 "fused_into_coefficients": list(self._FUSED_STAGES)}
         return (out.to_sparse_coo() if self.sparse else out), times, extra
 
     # ---- derivatives (soap_generator.py:1871-2699) ------------------------------------
@@ -841,38 +882,170 @@ class SOAP:
     def derivatives_numerical(self, system, centers=None, include=None, exclude=None, method="auto",
                               return_descriptor=True, attach=False, n_jobs=1, only_physical_cores=False,
                               verbose=False, d=1e-4):
-        """Central differences with fixed centers, float64 evaluations (soap_generator.py:1871-2003)."""
+        """Central differences with fixed centers, float64 evaluations (soap_generator.py:1871-2003).
+
+        The reference evaluates the whole system twice per (atom, component).  Here every
+        displacement of a batch of atoms runs in one launch sequence: each displaced copy
+        of an atom is appended to the atom array as an atom of its own, and each center
+        that the displacement can change (one within cutoff + 2d of the atom, or attached
+        to it) is evaluated once per displacement, with the other copies and the atom's
+        undisplaced position filtered out of its neighbour list.  A center the
+        displacement cannot reach has equal descriptors at +d and -d, so its row is zero
+        as in the reference.  Coefficients, power spectrum and differences are float64
+        with the exact radial functions; the result is cast to ``dtype``.
+        """
         if method not in ("auto", "numerical"):
             raise ValueError("method must be 'auto' or 'numerical'.")
         desc0 = self.create(system, centers) if return_descriptor else None
-        pos, Z, cell, pbc = system_to_arrays(system)
-        idx = self._included(pos.shape[0], include, exclude)
-        cxyz, _ = self.prepare_centers(pos, centers)
-        base = [cxyz[k].copy() for k in range(cxyz.shape[0])]
-        if attach:
-            attached = list(range(cxyz.shape[0])) if centers is None else \
-                [int(c) if isinstance(c, int) else -1 for c in centers]
-        else:
-            attached = [-1] * cxyz.shape[0]
-        orig = self.dtype
-        self.dtype = torch.float64
-        try:
-            deriv = torch.zeros((cxyz.shape[0], len(idx), 3, self.get_number_of_features()),
-                                dtype=torch.float64, device=self.device)
-            for ai, a in enumerate(idx):
-                for comp in range(3):
-                    fs = []
-                    for sgn in (1.0, -1.0):
-                        p = pos.copy()
-                        p[a, comp] += sgn * d
-                        sysd = {"positions": p, "atomic_numbers": Z, "cell": cell, "pbc": pbc}
-                        cen = [p[t] if t == a else b for t, b in zip(attached, base)]
-                        fs.append(self._evaluate(sysd, cen))
-                    deriv[:, ai, comp, :] = (fs[0] - fs[1]) / (2.0 * d)
-        finally:
-            self.dtype = orig
-        deriv = deriv.to(orig)
+        deriv = self._fd_batched(system, centers, include, exclude, attach, float(d))
+        deriv = deriv.to(self.dtype)
         return (deriv, desc0) if return_descriptor else deriv
+
+    # displaced evaluations per launch sequence of the batched finite differences
+    _FD_MAX_ROWS = 1 << 15
+
+    @torch.no_grad()
+    def _fd_batched(self, system, centers, include, exclude, attach, d) -> torch.Tensor:
+        pos, Z, cell, pbc = system_to_arrays(system)
+        n_atoms = pos.shape[0]
+        idx = self._included(n_atoms, include, exclude)
+        cxyz, _ = self.prepare_centers(pos, centers)
+        n_c = cxyz.shape[0]
+        if attach:
+            attached = np.arange(n_c, dtype=np.int64) if centers is None else \
+                np.array([int(c) if isinstance(c, int) else -1 for c in centers], dtype=np.int64)
+        else:
+            attached = -np.ones(n_c, dtype=np.int64)
+        n_feat = self.get_number_of_features()
+        deriv = torch.zeros((n_c, len(idx), 3, n_feat), dtype=torch.float64, device=self.device)
+        if not idx or n_atoms == 0:
+            return deriv
+        # the undisplaced system with the centers as fixed points: which (center, atom)
+        # pairs a displacement of d can change
+        base = self._context({"positions": pos, "atomic_numbers": Z, "cell": cell, "pbc": pbc},
+                             [cxyz[k] for k in range(n_c)])
+        nl = self._neighbors(base, drop_self=False, cutoff=self._cutoff + 2.0 * d)
+        counts = (nl.offsets[1:] - nl.offsets[:-1]).cpu().numpy()
+        nbr_atom = (nl.nbr[:nl.n_edges].to(torch.int64) % n_atoms).cpu().numpy()
+        pair_c = np.repeat(np.arange(n_c, dtype=np.int64), counts)
+        key = np.unique(pair_c * n_atoms + nbr_atom)
+        att = np.nonzero(attached >= 0)[0]
+        key = np.union1d(key, att * n_atoms + attached[att])
+        inc_pos = -np.ones(n_atoms, dtype=np.int64)
+        inc_pos[np.asarray(idx)] = np.arange(len(idx))
+        pc, pa = key // n_atoms, key % n_atoms
+        keep = inc_pos[pa] >= 0
+        pc, pa = pc[keep], pa[keep]
+        order = np.argsort(inc_pos[pa], kind="stable")        # pairs grouped by included atom
+        pc, pa = pc[order], pa[order]
+        # batches of whole atoms, at most _FD_MAX_ROWS displaced centers each
+        start = 0
+        n_pairs = pc.shape[0]
+        while start < n_pairs:
+            stop = start
+            while stop < n_pairs and (stop == start or 6 * (stop - start) < self._FD_MAX_ROWS):
+                a = pa[stop]
+                while stop < n_pairs and pa[stop] == a:
+                    stop += 1
+            self._fd_batch(pos, Z, cell, pbc, cxyz, attached, inc_pos, pc[start:stop], pa[start:stop], d, deriv)
+            start = stop
+        return deriv
+
+    def _fd_batch(self, pos, Z, cell, pbc, cxyz, attached, inc_pos, pc, pa, d, deriv):
+        """(f(+d) - f(-d)) / 2d of the (center pc[k], atom pa[k]) pairs, all 3 components."""
+        n_atoms = pos.shape[0]
+        atoms = np.unique(pa)
+        slot = -np.ones(n_atoms, dtype=np.int64)
+        slot[atoms] = np.arange(atoms.shape[0])
+        # displaced copy (atom, comp, sign) at index n_atoms + 6 slot + 2 comp + sign
+        disp = np.repeat(pos[atoms], 6, axis=0).reshape(-1, 6, 3)
+        for comp in range(3):
+            disp[:, 2 * comp, comp] = pos[atoms, comp] + d
+            disp[:, 2 * comp + 1, comp] = pos[atoms, comp] - d
+        disp = disp.reshape(-1, 3)
+        n_ext = n_atoms + disp.shape[0]
+        ext_pos = np.concatenate([pos, disp], 0)
+        ext_Z = np.concatenate([Z, np.repeat(Z[atoms], 6)])
+        # one displaced evaluation of center pc[k] per (comp, sign): row 6k + 2 comp + sign
+        P = pc.shape[0]
+        rows_c = np.repeat(pc, 6)
+        rows_a = np.repeat(pa, 6)
+        rows_v = n_atoms + 6 * np.repeat(slot[pa], 6) + np.tile(np.arange(6), P)
+        moved = attached[rows_c] == rows_a
+        rows_xyz = np.where(moved[:, None], ext_pos[rows_v], cxyz[rows_c])
+        x = self._fd_context(ext_pos, ext_Z, cell, pbc, rows_xyz)
+        # a displaced center snaps to an atom of its own displaced system (within 1e-12),
+        # as prepare_centers does, for its self term
+        x.self_sp = self._fd_self_species(x, n_atoms, rows_xyz, rows_a, rows_v)
+        nl = self._neighbors(x, drop_self=True)
+        nl = self._fd_filter(nl, n_ext, n_atoms, rows_a, rows_v)
+        coeffs = self._coefficients(x, nl, create_path=True, precision="float64", exact=True)
+        f = self._descriptor(coeffs, dtype=torch.float64, exact_spectrum=True).view(P, 3, 2, -1)
+        df = (f[:, :, 0, :] - f[:, :, 1, :]) / (2.0 * d)
+        ci = torch.from_numpy(pc).to(self.device)
+        ai = torch.from_numpy(inc_pos[pa]).to(self.device)
+        deriv[ci, ai] = df
+
+    def _fd_context(self, ext_pos, ext_Z, cell, pbc, rows_xyz) -> _Ctx:
+        x = _Ctx()
+        x.pos = self._dev(ext_pos)
+        x.n_atoms = ext_pos.shape[0]
+        x.centers = self._dev(np.ascontiguousarray(rows_xyz))
+        x.n_centers = rows_xyz.shape[0]
+        if self.periodic:
+            if cell is None or pbc is None:
+                raise ValueError("periodic=True requires cell and pbc.")
+            shifts = pbc_image_shifts(cell, pbc, self._cutoff)
+        else:
+            shifts = np.zeros((1, 3))
+        x.shifts = self._dev(shifts)
+        x.n_img = shifts.shape[0]
+        lut = np.full(max(self._species_Z + [int(ext_Z.max())]) + 1, -1, dtype=np.int32)
+        for z, i in self._Z_to_species_index.items():
+            lut[z] = i
+        x.sp_h = lut[ext_Z]
+        x.sp = self._dev(x.sp_h.astype(np.int32))
+        return x
+
+    def _fd_self_species(self, x, n_atoms, rows_xyz, rows_a, rows_v) -> torch.Tensor:
+        """Species of the atom each displaced center coincides with (argmin, d^2 <= 1e-24), or -1.
+
+        The displaced system of row v is the undisplaced atoms with atom rows_a[v] at the
+        position of its copy rows_v[v]."""
+        pos = x.pos[:n_atoms]
+        q = x.centers
+        a = torch.from_numpy(rows_a).to(self.device)
+        v = torch.from_numpy(rows_v).to(self.device)
+        sp = x.sp[:n_atoms].to(torch.int64)
+        out = torch.empty(q.shape[0], dtype=torch.int32, device=self.device)
+        step = max(1, (1 << 23) // max(n_atoms, 1))
+        for s0 in range(0, q.shape[0], step):
+            s1 = min(q.shape[0], s0 + step)
+            d2 = ((pos[None, :, :] - q[s0:s1, None, :]) ** 2).sum(-1)
+            da = ((x.pos[v[s0:s1]] - q[s0:s1]) ** 2).sum(-1)
+            d2.scatter_(1, a[s0:s1, None], da[:, None])
+            m, j = d2.min(1)
+            out[s0:s1] = torch.where(m <= 1e-24, sp[j], torch.full_like(j, -1)).to(torch.int32)
+        return out
+
+    def _fd_filter(self, nl: _NL, n_ext, n_atoms, rows_a, rows_v) -> _NL:
+        """Drop from row v's edges every image of atom rows_a[v] and of the copies other than rows_v[v]."""
+        n_rows = nl.offsets.shape[0] - 1
+        e = nl.nbr[:nl.n_edges].to(torch.int64)
+        cnt = nl.offsets[1:] - nl.offsets[:-1]
+        row = torch.repeat_interleave(torch.arange(n_rows, device=self.device), cnt)
+        atom = e % n_ext
+        a = torch.from_numpy(rows_a).to(self.device)[row]
+        v = torch.from_numpy(rows_v).to(self.device)[row]
+        keep = (atom != a) & ((atom < n_atoms) | (atom == v))
+        nbr = nl.nbr[:nl.n_edges][keep].contiguous()
+        cnt = torch.bincount(row[keep], minlength=n_rows)
+        offsets = torch.zeros(n_rows + 1, dtype=torch.int64, device=self.device)
+        offsets[1:] = torch.cumsum(cnt, 0)
+        n_edges = int(nbr.shape[0])
+        if n_edges == 0:
+            nbr = torch.zeros(1, dtype=torch.int32, device=self.device)
+        return _NL(offsets, nbr, n_edges, int(cnt.max().item()) if n_rows else 0)
 
     def derivatives(self, system, centers=None, include=None, exclude=None, method="auto",
                     return_descriptor=True, attach=False, n_jobs=1, only_physical_cores=False, verbose=False):
