@@ -372,7 +372,8 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             const double cx = gcen[ci][0], cy = gcen[ci][1], cz = gcen[ci][2];
             const int cslot = gslot[ci];
             const int ext = nbr[e];
-            const int img = ext / n_atoms, a = ext - img * n_atoms;
+            int img, a;
+            split_ext(ext, n_atoms, 1.0 / (double)n_atoms, img, a);
             const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
             const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
             const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
@@ -630,7 +631,8 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
         const int64_t e = base + lane;
         if (e < e1) {
             const int ext = nbr[e];
-            const int img = ext / n_atoms, a = ext - img * n_atoms;
+            int img, a;
+            split_ext(ext, n_atoms, 1.0 / (double)n_atoms, img, a);
             const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
             const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
             const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;

@@ -183,6 +183,11 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
     const int pc = lane & 31, pc_l = pc / n, pc_n = pc - (pc / n) * n, phalf = lane >> 5;
     const double tab_inv_h = RBF == kRbfTable ? b.rad_intervals / b.rad_rmax : 0.0;
     const TA tab_h = RBF == kRbfTable ? (TA)(b.rad_rmax / b.rad_intervals) : TA(0);
+    // every row slot holds a valid table offset from the start, so the loads of slots
+    // past a wavefront's positions (whose results phase B never reads) stay in bounds
+    if (RBF == kRbfTable)
+        for (int i = tid; i < kTE; i += kThreads) tab_row[i] = 0;
+    const double inv_na = 1.0 / (double)n_atoms;
     __syncthreads();
 
     for (int s0 = 0; s0 < S; s0 += SB) {
@@ -228,8 +233,8 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                 int sp = -1;
                 double rx = 0.0, ry = 0.0, rz = 0.0, r2 = 0.0, r = 0.0, w = 1.0;
                 if (lane < nt) {
-                    const int ext = ext_cur;
-                    const int img = ext / n_atoms, a = ext - img * n_atoms;
+                    int img, a;
+                    split_ext(ext_cur, n_atoms, inv_na, img, a);
                     rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
                     ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
                     rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
@@ -276,39 +281,39 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                         hw[1] = tw * tab_h * (t3 - TA(2) * t2 + t);
                         hw[2] = tw * (TA(3) * t2 - TA(2) * t3);
                         hw[3] = tw * tab_h * (t3 - t2);
-                        rq[pidx >> 2] = ((cslot * S + sp) * (G + 1) + i) * NS;
+                        rq[pidx >> 2] = (int32_t)(4u * (uint32_t)(((cslot * S + sp) * (G + 1) + i) * NS));   // bytes
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     const int nk = (nv - q + 3) >> 2;       // this wavefront's positions
-                    for (int k0 = 0; k0 < nk; k0 += 4) {
-                        const int ne = nk - k0 < 4 ? nk - k0 : 4;
-                        const int4 rows = *(const int4*)(rq + k0);
-                        const int rw[4] = {rows.x, ne > 1 ? rows.y : rows.x, ne > 2 ? rows.z : rows.x,
-                                           ne > 3 ? rows.w : rows.x};
-                        if (ch_pair) {
+                    // loads address the table as its uniform base plus a 32-bit byte offset
+                    const char* tb = (const char*)b.rad_table;
+                    const uint32_t ns_b = 4u * (uint32_t)NS;
+                    if (ch_pair) {
+                        const bool on = pc < L1n;
+                        const uint32_t coff = on ? 8u * (uint32_t)(pc_l * n + pc_n) : 0u;
+                        TA* rb = Rs + q * RS + pc_l * nP + pc_n;
+                        for (int k0 = 0; k0 < nk; k0 += 4) {
                             // half h: slots j = h and j = h + 2
-                            const bool on = pc < L1n;
-                            const int coff = on ? (pc_l * n + pc_n) * 2 : 0;
-                            const int ra = phalf ? rw[1] : rw[0], rb = phalf ? rw[3] : rw[2];
-                            const float* aa = b.rad_table + (uint32_t)(ra + coff);
-                            const float* ab = b.rad_table + (uint32_t)(rb + coff);
-                            const float2 va0 = *(const float2*)aa, va1 = *(const float2*)(aa + NS);
-                            const float2 vb0 = *(const float2*)ab, vb1 = *(const float2*)(ab + NS);
+                            const int4 rows = *(const int4*)(rq + k0);
+                            const uint32_t oa = (uint32_t)(phalf ? rows.y : rows.x) + coff;
+                            const uint32_t ob = (uint32_t)(phalf ? rows.w : rows.z) + coff;
+                            const float2 va0 = *(const float2*)(tb + oa), va1 = *(const float2*)(tb + (oa + ns_b));
+                            const float2 vb0 = *(const float2*)(tb + ob), vb1 = *(const float2*)(tb + (ob + ns_b));
                             if (on) {
                                 const int ja = k0 + phalf, jb = k0 + phalf + 2;
                                 const typename V2<TA>::type wa01 = *(const typename V2<TA>::type*)(wq + ja * 4);
                                 const typename V2<TA>::type wa23 = *(const typename V2<TA>::type*)(wq + ja * 4 + 2);
                                 const typename V2<TA>::type wb01 = *(const typename V2<TA>::type*)(wq + jb * 4);
                                 const typename V2<TA>::type wb23 = *(const typename V2<TA>::type*)(wq + jb * 4 + 2);
-                                Rs[(q + 4 * ja) * RS + pc_l * nP + pc_n] = wa01.x * (TA)va0.x + wa01.y * (TA)va0.y +
-                                                                          wa23.x * (TA)va1.x + wa23.y * (TA)va1.y;
-                                Rs[(q + 4 * jb) * RS + pc_l * nP + pc_n] = wb01.x * (TA)vb0.x + wb01.y * (TA)vb0.y +
-                                                                          wb23.x * (TA)vb1.x + wb23.y * (TA)vb1.y;
+                                rb[ja * 4 * RS] = wa01.x * (TA)va0.x + wa01.y * (TA)va0.y +
+                                                          wa23.x * (TA)va1.x + wa23.y * (TA)va1.y;
+                                rb[jb * 4 * RS] = wb01.x * (TA)vb0.x + wb01.y * (TA)vb0.y +
+                                                          wb23.x * (TA)vb1.x + wb23.y * (TA)vb1.y;
                             }
-                            continue;
                         }
+                    } else {
                         for (int ch0 = 0; ch0 < L1n; ch0 += 64) {
                             const bool on = ch0 + lane < L1n;
                             int ll = ch_l, nn = ch_n;
@@ -316,23 +321,29 @@ __global__ __launch_bounds__(kThreads, coeff_min_waves<PREC>()) void coeff_kerne
                                 ll = (ch0 + lane) / n;
                                 nn = ch0 + lane - ll * n;
                             }
-                            const int coff = on ? (ll * n + nn) * 2 : 0;
-                            float2 v0[4], v1[4];
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                const float* a0 = b.rad_table + (uint32_t)(rw[j] + coff);
-                                v0[j] = *(const float2*)a0;
-                                v1[j] = *(const float2*)(a0 + NS);
-                            }
-                            // slots j >= ne (k0 + j < 16, so position < 64) fill rows that phase B
-                            // never reads: no per-slot branch
-                            if (on) {
+                            const uint32_t coff = on ? 8u * (uint32_t)(ll * n + nn) : 0u;
+                            TA* rb = Rs + q * RS + ll * nP + nn;
+                            for (int k0 = 0; k0 < nk; k0 += 4) {
+                                const int4 rows = *(const int4*)(rq + k0);
+                                const uint32_t o[4] = {(uint32_t)rows.x + coff, (uint32_t)rows.y + coff,
+                                                       (uint32_t)rows.z + coff, (uint32_t)rows.w + coff};
+                                float2 v0[4], v1[4];
 #pragma unroll
                                 for (int j = 0; j < 4; ++j) {
-                                    const typename V2<TA>::type hw01 = *(const typename V2<TA>::type*)(wq + (k0 + j) * 4);
-                                    const typename V2<TA>::type hw23 = *(const typename V2<TA>::type*)(wq + (k0 + j) * 4 + 2);
-                                    Rs[(q + 4 * (k0 + j)) * RS + ll * nP + nn] = hw01.x * (TA)v0[j].x + hw01.y * (TA)v0[j].y +
-                                                                                 hw23.x * (TA)v1[j].x + hw23.y * (TA)v1[j].y;
+                                    v0[j] = *(const float2*)(tb + o[j]);
+                                    v1[j] = *(const float2*)(tb + (o[j] + ns_b));
+                                }
+                                // slots j >= ne (k0 + j < 16, so position < 64) fill rows that phase B
+                                // never reads: no per-slot branch
+                                if (on) {
+                                    TA* rr = rb + k0 * 4 * RS;
+#pragma unroll
+                                    for (int j = 0; j < 4; ++j) {
+                                        const typename V2<TA>::type hw01 = *(const typename V2<TA>::type*)(wq + (k0 + j) * 4);
+                                        const typename V2<TA>::type hw23 = *(const typename V2<TA>::type*)(wq + (k0 + j) * 4 + 2);
+                                        rr[j * 4 * RS] = hw01.x * (TA)v0[j].x + hw01.y * (TA)v0[j].y +
+                                                                 hw23.x * (TA)v1[j].x + hw23.y * (TA)v1[j].y;
+                                    }
                                 }
                             }
                         }
@@ -612,8 +623,8 @@ int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int 
     if (create_path && !self_sp) return fail(CUSOAP_ERR_ARG, "coefficients: create path needs center_self_species");
     if (b->rad_table && (!self_sp || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
         return fail(CUSOAP_ERR_ARG, "coefficients: radial table needs center_self_species and a valid grid");
-    if (b->rad_table && (double)(b->n_species + 1) * b->n_species * (b->rad_intervals + 1.0) * (b->l_max + 1) * 2 * b->n_max >= 2147483647.0)
-        return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: radial table larger than 2^31 values");
+    if (b->rad_table && (double)(b->n_species + 1) * b->n_species * (b->rad_intervals + 1.0) * (b->l_max + 1) * 2 * b->n_max >= 1073741824.0)
+        return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: radial table larger than 2^30 values");
     if (n_centers == 0) return CUSOAP_OK;
 #define CUSOAP_CASE(L) \
     case L: return coefficients_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr, create_path, out, st);

@@ -43,6 +43,22 @@ constexpr int kMaxS = 64;       // species
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Extended index ext = img * n_atoms + atom (0 <= ext < 2^31) split without an
+// integer division.  inv_n = 1.0 / n_atoms in float64: ext * inv_n has a relative
+// error below 2^-51, so for ext < 2^31 it never reaches the next integer above
+// ext / n_atoms, and it truncates to the true quotient except at exact multiples,
+// where it can be one less; the remainder check corrects that case.
+__device__ __forceinline__ void split_ext(int ext, int n_atoms, double inv_n, int& img, int& atom) {
+    int q = (int)((double)ext * inv_n);
+    int r = ext - q * n_atoms;
+    if (r >= n_atoms) {
+        ++q;
+        r -= n_atoms;
+    }
+    img = q;
+    atom = r;
+}
+
 // ---------------------------------------------------------------------------
 // real solid harmonics r^l Y_lm(r) and Cartesian gradients
 // ---------------------------------------------------------------------------

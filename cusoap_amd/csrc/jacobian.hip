@@ -189,7 +189,8 @@ __global__ __launch_bounds__(256) void records_kernel(cusoap_basis_t b, int mode
         if (tid < nt) {
             const int64_t e = t0 + tid;
             const int ext = nbr[e];
-            const int img = ext / n_atoms, a = ext - img * n_atoms;
+            int img, a;
+            split_ext(ext, n_atoms, 1.0 / (double)n_atoms, img, a);
             const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
             const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
             const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
