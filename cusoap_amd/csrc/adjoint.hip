@@ -28,6 +28,8 @@
 // float64 tile kernel.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -231,6 +233,114 @@ __global__ __launch_bounds__(256) void ps_adjoint_shared_kernel(int S, int n, in
         }
         __syncthreads();
     }
+}
+
+// Power-spectrum adjoint on the matrix cores, float32 coefficients and M with no
+// beta folded in (the mixed and radial-table forms).  Per center and degree l,
+// G^T[m][a] = sum_b c[b][l^2 + m] W_l[a][b] is a (2l+1) x SN x SN product:
+// v_mfma_f32_16x16x4_f32 in K steps of 4 over b (float32 products and float32 sums,
+// the arithmetic of the vector kernel in another order), rows m in tiles of 16.
+// Each wavefront takes one center at a time: its coefficient block is copied to the
+// wavefront's LDS slice with coalesced reads, the products read A (coefficients)
+// and B = W_l^T (staged once per workgroup) from LDS, the results are scattered
+// into the [s][lm][k] order of M in a second LDS slice (with K_lm when fold_k), and
+// the block is written out with coalesced stores.  Lane layout of the instruction:
+// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4),
+// element i % 4.
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+__host__ __device__ inline int ps_mfma_slice(int per) { return (per + 3) & ~3; }
+constexpr int kPsMfmaPre = 32;           // prefetched values per lane (blocks up to 2048 values)
+constexpr int kPsMfmaMaxNT = 4;          // output tiles of 16 (species x n_max up to 64)
+__global__ __launch_bounds__(256) void ps_adjoint_mfma_kernel(int S, int n, int L1, const float* __restrict__ coeffs,
+                                                              const float* __restrict__ Wg, int fold_k, int n_centers,
+                                                              float* __restrict__ M) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int SN = S * n, L2 = L1 * L1, per = SN * L2, slice = ps_mfma_slice(per);
+    const int RS = psw_row(SN), LS = psw_deg(SN), nw = psw_size(L1, SN);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    float* W = (float*)smem;                       // padded [L1][SN][SN] + 3 zero rows
+    float* knorm = W + ((nw + 3) & ~3);            // [L2] K_lm (or 1)
+    float* cin = knorm + ((L2 + 3) & ~3) + wv * 2 * slice;   // this wavefront's coefficient block
+    float* cout = cin + slice;                     // and its M block
+    for (int i = tid; i < nw; i += 256) W[i] = Wg[i];
+    for (int i = tid; i < L2; i += 256) knorm[i] = fold_k ? (float)sh_norm_lm(i) : 1.f;
+    __syncthreads();
+    const int NT = (SN + 15) >> 4, KS = (SN + 3) >> 2;
+    const int i16 = lane & 15, kk = lane >> 4;
+    // this lane's output a = 16 nt + i16 as (species, k) offsets into the M block
+    int aoff[kPsMfmaMaxNT];
+#pragma unroll
+    for (int nt = 0; nt < kPsMfmaMaxNT; ++nt) {
+        const int a = 16 * nt + i16, sa = a / n;
+        aoff[nt] = a < SN ? sa * L2 * n + (a - sa * n) : -1;
+    }
+    const int stride = gridDim.x * 4;
+    // the next center's block is loaded into registers while this one is processed
+    float pre[kPsMfmaPre];
+    auto fetch = [&](int c) {
+        if (c >= n_centers) return;
+        const float* src = coeffs + (size_t)c * per;
+#pragma unroll
+        for (int j = 0; j < kPsMfmaPre; ++j) {
+            const int i = lane + 64 * j;
+            if (i < per) pre[j] = src[i];
+        }
+    };
+    fetch(blockIdx.x * 4 + wv);
+    for (int c = blockIdx.x * 4 + wv; c < n_centers; c += stride) {
+#pragma unroll
+        for (int j = 0; j < kPsMfmaPre; ++j) {
+            const int i = lane + 64 * j;
+            if (i < per) cin[i] = pre[j];
+        }
+        fetch(c + stride);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int l = 0; l < L1; ++l) {
+            const int d = 2 * l + 1, l2 = l * l;
+#pragma unroll
+            for (int nt = 0; nt < kPsMfmaMaxNT; ++nt) {
+                if (nt >= NT) break;
+                const int a = 16 * nt + i16;       // this lane's B column / output a
+                const bool a_ok = a < SN;
+                const float* Wa = W + l * LS + a * RS;
+                const int ao = aoff[nt];
+                for (int m0 = 0; m0 < d; m0 += 16) {
+                    const int mr = m0 + i16;       // this lane's A row
+                    mf32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    for (int ks = 0; ks < KS; ++ks) {
+                        const int bb = 4 * ks + kk;
+                        const bool b_ok = bb < SN;
+                        const float av = (mr < d && b_ok) ? cin[bb * L2 + l2 + mr] : 0.f;
+                        const float bv = (a_ok && b_ok) ? Wa[bb] : 0.f;
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+                    }
+                    if (a_ok) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int m = m0 + 4 * kk + v;
+                            if (m < d) cout[ao + (l2 + m) * n] = acc[v] * knorm[l2 + m];
+                        }
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float* dst = M + (size_t)c * per;
+        for (int i = lane; i < per; i += 64) dst[i] = cout[i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// CUSOAP_PS_ADJOINT=vector selects the vector kernel instead (read at each call)
+inline bool ps_mfma_enabled() {
+    const char* v = std::getenv("CUSOAP_PS_ADJOINT");
+    return !(v && std::strcmp(v, "vector") == 0);
 }
 
 template <typename T>
@@ -779,6 +889,30 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
     return CUSOAP_OK;
 }
 
+template <typename TG>
+int launch_ps_mfma(const cusoap_basis_t& b, const void* coeffs, const int32_t* pair_feature, const void* gd,
+                   int fold_k, int n_centers, void* M, void* Wbuf, size_t smem, hipStream_t st) {
+    const int L1 = b.l_max + 1, SN = b.n_species * b.n_max;
+    const int nw = psw_size(L1, SN);
+    prof_begin("vjp_adjoint", st);
+    ps_weights_kernel<TG, float><<<(nw + 255) / 256, 256, 0, st>>>(L1, SN, pair_feature, (const TG*)gd, (float*)Wbuf);
+    CUSOAP_LAUNCH_CHECK();
+    auto kern = ps_adjoint_mfma_kernel;
+    if (smem > 64 * 1024)
+        CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    // one resident round of workgroups (every wavefront takes the same number of centers)
+    int dev = 0, n_cu = 0, per_cu = 0;
+    CUSOAP_HIP(hipGetDevice(&dev));
+    CUSOAP_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    CUSOAP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, 256, smem));
+    const int grid = std::max(1, std::min((n_centers + 3) / 4, std::max(1, n_cu * per_cu)));
+    kern<<<grid, 256, smem, st>>>(b.n_species, b.n_max, L1, (const float*)coeffs, (const float*)Wbuf, fold_k,
+                                  n_centers, (float*)M);
+    prof_end("vjp_adjoint", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
 template <typename TC, typename TG, typename TW, typename TM>
 int launch_ps_shared(const cusoap_basis_t& b, const void* coeffs, const int32_t* pair_feature, const void* gd,
                      int fold_beta, int fold_k, int n_centers, void* M, void* Wbuf, size_t smem, hipStream_t st) {
@@ -819,6 +953,14 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
         const size_t tw = w64 ? 8 : 4;
         const size_t smem = tw * ((size_t)psw_size(L1, SN) + 2 * (size_t)per + (fold_beta ? (size_t)L1 * b.n_max * b.n_max : 0));
         void* Wbuf = (char*)M + vjp_m_bytes(&b, n_centers);
+        if constexpr (std::is_same<TC, float>::value && std::is_same<TM, float>::value) {
+            if (!fold_beta && per <= 64 * kPsMfmaPre && SN <= 16 * kPsMfmaMaxNT && ps_mfma_enabled()) {
+                const size_t smem_m = sizeof(float) * ((size_t)((psw_size(L1, SN) + 3) & ~3) + ((L1 * L1 + 3) & ~3) +
+                                                       8 * (size_t)ps_mfma_slice(per));
+                if (smem_m <= 160 * 1024)
+                    return launch_ps_mfma<TG>(b, coeffs, pair_feature, gd, fold_k, n_centers, M, Wbuf, smem_m, st);
+            }
+        }
         if (smem <= 160 * 1024 && per <= 256 * kPsPre)
             return w64 ? launch_ps_shared<TC, TG, double, TM>(b, coeffs, pair_feature, gd, fold_beta, fold_k, n_centers, M, Wbuf, smem, st)
                        : launch_ps_shared<TC, TG, float, TM>(b, coeffs, pair_feature, gd, fold_beta, fold_k, n_centers, M, Wbuf, smem, st);
