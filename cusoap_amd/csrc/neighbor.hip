@@ -6,9 +6,9 @@
 //   count  : bin every image atom (ext = img * N + atom) that falls in the grid
 //            (counting sort), then one wavefront per center scans the 27
 //            surrounding cells, counts edges with a ballot and, up to 384 edges,
-//            stores them in order in the workspace (each at its rank among the
-//            center's ext indices, which are distinct): the order of the
-//            reference's `nonzero` over (center, ext)
+//            stores them in the workspace sorted by ext index (the order of the
+//            reference's `nonzero` over (center, ext)) with a bitonic network in
+//            the wavefront's registers
 //   scan   : exclusive scan of the counts -> CSR offsets
 //   fill   : a copy of the sorted rows to the CSR ranges; a center with more
 //            than 384 edges is searched again by a workgroup of its own, which
@@ -416,6 +416,42 @@ __global__ __launch_bounds__(256) void fill_kernel(cusoap_nl_plan_t plan, const 
     }
 }
 
+// Ascending bitonic sort of the 64 KPL values v[t] (element e = 64 t + lane) of one
+// wavefront.  Stage (k, j) pairs e with e ^ j; the lower element keeps the minimum when
+// e & k == 0.  Strides below 64 exchange across lanes (__shfl_xor), larger ones between
+// a lane's own registers.
+template <int KPL>
+__device__ __forceinline__ void bitonic_wave(int (&v)[KPL], int lane) {
+    constexpr int N = 64 * KPL;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int tj = j >> 6;
+#pragma unroll
+                for (int t = 0; t < KPL; ++t) {
+                    if (t & tj) continue;              // t is the lower element of (t, t | tj)
+                    const bool asc = ((64 * t + lane) & k) == 0;
+                    const int a = v[t], b = v[t | tj];
+                    const int lo = a < b ? a : b, hi = a < b ? b : a;
+                    v[t] = asc ? lo : hi;
+                    v[t | tj] = asc ? hi : lo;
+                }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int t = 0; t < KPL; ++t) {
+                    const bool asc = ((64 * t + lane) & k) == 0;
+                    const int o = __shfl_xor(v[t], j, 64);
+                    const int mn = v[t] < o ? v[t] : o, mx = v[t] < o ? o : v[t];
+                    v[t] = (lower == asc) ? mn : mx;
+                }
+            }
+        }
+    }
+}
+
 // Count pass, one wavefront per center (4 centers per workgroup): counts the
 // center's edges and, when there are at most CAP, also puts them in order (the list
 // fill_kernel makes) in the center's row of `sorted`.  The passing candidates are
@@ -454,42 +490,31 @@ __global__ __launch_bounds__(256) void count_sort_kernel(cusoap_nl_plan_t plan, 
     }
     if (lane == 0) counts[c] = n;
     if (n > CAP) return;            // the fill pass searches this center again
-    const int n4 = (n + 3) & ~3;
-    if (lane < n4 - n) keys[n + lane] = INT_MAX;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int32_t* dst = sorted + (size_t)c * CAP;
-    // each lane holds its keys i = lane + 64 t in registers and ranks all of them in one
-    // pass over the keys (one 16-byte LDS read serves every key of the lane)
+    // the keys i = lane + 64 t, padded with INT_MAX to 64 KPL (a power of two), sorted by
+    // a bitonic network in registers; the row is then stored in element order
     auto place = [&](auto kpl_c) {
         constexpr int KPL = decltype(kpl_c)::value;
-        int kk[KPL], rk[KPL];
+        int kk[KPL];
 #pragma unroll
         for (int t = 0; t < KPL; ++t) {
             const int i = lane + 64 * t;
             kk[t] = i < n ? keys[i] : INT_MAX;
-            rk[t] = 0;
         }
-        for (int j = 0; j < n4; j += 4) {
-            const int4 q = *(const int4*)(keys + j);
-#pragma unroll
-            for (int t = 0; t < KPL; ++t) rk[t] += (q.x < kk[t]) + (q.y < kk[t]) + (q.z < kk[t]) + (q.w < kk[t]);
-        }
+        bitonic_wave<KPL>(kk, lane);
 #pragma unroll
         for (int t = 0; t < KPL; ++t)
-            if (lane + 64 * t < n) dst[rk[t]] = kk[t];
+            if (lane + 64 * t < n) dst[lane + 64 * t] = kk[t];
     };
-    static_assert(CAP <= 6 * 64, "count_sort_kernel: at most 6 keys per lane");
-    switch ((n + 63) >> 6) {
-        case 0: break;
-        case 1: place(std::integral_constant<int, 1>()); break;
-        case 2: place(std::integral_constant<int, 2>()); break;
-        case 3: place(std::integral_constant<int, 3>()); break;
-        case 4: place(std::integral_constant<int, 4>()); break;
-        case 5: place(std::integral_constant<int, 5>()); break;
-        default: place(std::integral_constant<int, 6>()); break;
-    }
+    static_assert(CAP <= 8 * 64, "count_sort_kernel: at most 8 keys per lane");
+    const int nt = (n + 63) >> 6;
+    if (nt == 1) place(std::integral_constant<int, 1>());
+    else if (nt == 2) place(std::integral_constant<int, 2>());
+    else if (nt <= 4) place(std::integral_constant<int, 4>());
+    else if (nt > 4) place(std::integral_constant<int, 8>());
 }
 
 // Fill pass when every center has at most kSortCap edges: one wavefront per center
