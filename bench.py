@@ -58,7 +58,7 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="water_box", choices=["water_box", "hcno_box", "h2o_1m", "h2o_1000"])
     ap.add_argument("--precision", default="mixed", choices=["float64", "mixed", "float32"])
-    ap.add_argument("--radial", default="learned", choices=["exact", "table", "learned"],
+    ap.add_argument("--radial", default="learned", choices=["exact", "table", "learned", "chebyshev"],
                     help="radial functions of the timed form: exact per edge, the tabulated exact functions, "
                          "or the learned model (trained and gated first)")
     ap.add_argument("--radial-checkpoint", default="", metavar="NPZ",

@@ -30,7 +30,8 @@ class Basis(C.Structure):
                 ("w_r0", f64), ("w_c", f64), ("w_d", f64), ("w_m", f64), ("w0", f64),
                 ("gto_gamma", p), ("gto_pref", p), ("gto_beta", p),
                 ("poly_rx", p), ("poly_rxw", p), ("poly_gss", p), ("poly_B0", p), ("poly_Bh0", p),
-                ("self_l0", p), ("rad_table", p), ("rad_intervals", i32), ("rad_rmax", f64)]
+                ("self_l0", p), ("rad_table", p), ("rad_intervals", i32), ("rad_rmax", f64),
+                ("rad_cheb", p), ("cheb_terms", i32), ("cheb_split", f64)]
 
 
 class NLPlan(C.Structure):

@@ -612,6 +612,10 @@ CUSOAP_COEFF_DECL(5) CUSOAP_COEFF_DECL(6) CUSOAP_COEFF_DECL(7) CUSOAP_COEFF_DECL
 #define CUSOAP_COEFF_DEF2(L) CUSOAP_COEFF_DEF(L)
 CUSOAP_COEFF_DEF2(CUSOAP_ONLY_L)
 #else
+int cheb_coefficients_impl(const cusoap_basis_t* b, const double* pos, int n_atoms, const int32_t* atom_sp,
+                           const double* shifts, const double* centers, int n_centers, const int32_t* self_sp,
+                           const int64_t* offsets, const int32_t* nbr, int create_path, float* out, hipStream_t st);
+
 int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
                       const double* shifts, int n_img, const double* centers, int n_centers,
                       const int32_t* self_sp, const int64_t* offsets, const int32_t* nbr, int create_path,
@@ -621,6 +625,12 @@ int coefficients_impl(const cusoap_basis_t* b, int prec, const double* pos, int 
         return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: n_max must be 1..16 and l_max 0..9");
     if (prec < 0 || prec > 2) return fail(CUSOAP_ERR_ARG, "coefficients: unknown precision");
     if (create_path && !self_sp) return fail(CUSOAP_ERR_ARG, "coefficients: create path needs center_self_species");
+    if (b->rad_cheb) {
+        if (prec == CUSOAP_PREC_F64)
+            return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: the Chebyshev radial form computes in float32");
+        return cheb_coefficients_impl(b, pos, n_atoms, atom_sp, shifts, centers, n_centers, self_sp, offsets, nbr,
+                                      create_path, (float*)out, st);
+    }
     if (b->rad_table && (!self_sp || b->rad_intervals < 1 || !(b->rad_rmax > 0.0)))
         return fail(CUSOAP_ERR_ARG, "coefficients: radial table needs center_self_species and a valid grid");
     if (b->rad_table && (double)(b->n_species + 1) * b->n_species * (b->rad_intervals + 1.0) * (b->l_max + 1) * 2 * b->n_max >= 1073741824.0)

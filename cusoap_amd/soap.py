@@ -274,14 +274,15 @@ class SOAP:
         self._basis_learned = None
         self._species_cache = None   # device tensors derived from (Z, cell, pbc) of the last system
         self._last_nl_backend = "hip-cell-list"
-        if radial not in ("exact", "table", "learned"):
-            raise ValueError("radial must be 'exact', 'table' or 'learned'")
+        if radial not in ("exact", "table", "learned", "chebyshev"):
+            raise ValueError("radial must be 'exact', 'table', 'learned' or 'chebyshev'")
         if spectrum not in _SPECTRA:
             raise ValueError(f"spectrum must be one of {sorted(_SPECTRA)}")
         self.spectrum = spectrum
         self.radial = radial
         self.surrogate = None
         self._rad_table = None
+        self._rad_cheb = None
         if radial == "learned":
             self._init_learned_radial(surrogate, int(table_intervals))
         elif radial == "table":
@@ -291,6 +292,8 @@ class SOAP:
                 raise NotImplementedError("radial='table' covers the GTO basis")
             self._rad_intervals = int(table_intervals)
             self._rad_table = radial_table(self, self._rad_intervals)
+        elif radial == "chebyshev":
+            self._init_chebyshev_radial()
 
     def _sibling(self, **overrides) -> "SOAP":
         """A SOAP of the same basis (same Loewdin matrices) with some arguments replaced."""
@@ -326,6 +329,30 @@ class SOAP:
         exact = radial_table(self, n_intervals)
         net = radial_table(self, n_intervals, model=model, use_pair=self._learned_use_pair)
         self._rad_table = masked_table(exact, net, channels)
+
+    def _init_chebyshev_radial(self, coefficients=None):
+        """Chebyshev series of the radial functions for the matrix-core kernels (exact
+        functions, or the given per-slot coefficients (a, d, K))."""
+        from . import chebyshev as cheb
+        self._require_gpu()
+        why = cheb.supported(self)
+        if why is not None:
+            raise NotImplementedError(why)
+        if self.precision == "float64":
+            raise ValueError("the Chebyshev radial form computes in float32 (fp16 hi/lo matrix-core products): "
+                             "use precision='mixed'")
+        if coefficients is None:
+            a, d, ef, ed = cheb.fit_exact(self)
+            if ef > cheb.TOL_VALUE or ed > cheb.TOL_SLOPE:
+                raise NotImplementedError(f"Chebyshev radial form: {cheb.TERMS} terms per interval fit this basis to "
+                                          f"{ef:.1e} (values) and {ed:.1e} (slopes) only; use radial='table'")
+            self._cheb_fit_err = (ef, ed)
+        else:
+            a, d = coefficients
+        self._cheb_terms = cheb.TERMS
+        self._cheb_split = cheb.split_radius(self._cutoff)
+        self._rad_cheb = cheb.device_table(self, a, d)
+        self._basis_learned = None
 
     # ---- basis (soap_generator.py:690-819) ---------------------------------
     def _init_gto_basis(self, betas=None):
@@ -487,13 +514,18 @@ class SOAP:
     def _basis(self) -> "_lib.Basis":
         """Basis tables for the kernels, with the radial table when radial='learned'."""
         b = self._basis_exact()
-        if self._rad_table is None:
+        if self._rad_table is None and self._rad_cheb is None:
             return b
         if self._basis_learned is None:
             bl = _lib.Basis.from_buffer_copy(b)
-            bl.rad_table = self._rad_table.data_ptr()
-            bl.rad_intervals = self._rad_intervals
-            bl.rad_rmax = self._cutoff
+            if self._rad_cheb is not None:
+                bl.rad_cheb = self._rad_cheb.data_ptr()
+                bl.cheb_terms = self._cheb_terms
+                bl.cheb_split = self._cheb_split
+            else:
+                bl.rad_table = self._rad_table.data_ptr()
+                bl.rad_intervals = self._rad_intervals
+                bl.rad_rmax = self._cutoff
             self._basis_learned = bl
         return self._basis_learned
 

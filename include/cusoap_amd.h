@@ -78,6 +78,17 @@ typedef struct cusoap_basis {
     const float* rad_table;
     int32_t rad_intervals;
     double rad_rmax;
+    /* Chebyshev radial model (radial="chebyshev", and the learned model in that form),
+     * float32 device array or NULL.  Two intervals [0, cheb_split] and [cheb_split,
+     * cutoff], cheb_terms (32) terms each.  Layout ((S+1) S slots, 2, l_max+1, 2
+     * intervals, cheb_terms, 8): the coefficients a_j of f_nl(r) = r^l B_nl(r) (index 0)
+     * and of df_nl/dr (index 1) in T_j(t), t = 2 (r - lo) / (hi - lo) - 1 on the interval
+     * [lo, hi] holding r, n padded to 8, slots as for rad_table.  When set, the float
+     * coefficient and adjoint stages run on the matrix cores (fp16 hi/lo operands,
+     * float32 accumulation) and rad_table is not read. */
+    const float* rad_cheb;
+    int32_t cheb_terms;
+    double cheb_split;
 } cusoap_basis_t;
 
 /* Cell-list geometry of one neighbour search. */
