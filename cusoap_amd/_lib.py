@@ -31,7 +31,8 @@ class Basis(C.Structure):
                 ("gto_gamma", p), ("gto_pref", p), ("gto_beta", p),
                 ("poly_rx", p), ("poly_rxw", p), ("poly_gss", p), ("poly_B0", p), ("poly_Bh0", p),
                 ("self_l0", p), ("rad_table", p), ("rad_intervals", i32), ("rad_rmax", f64),
-                ("rad_cheb", p), ("cheb_terms", i32), ("cheb_split", f64)]
+                ("rad_cheb", p), ("cheb_terms", i32), ("cheb_split", f64),
+                ("cheb_frag", p), ("cheb_rowinv", p)]
 
 
 class NLPlan(C.Structure):

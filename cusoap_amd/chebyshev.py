@@ -166,6 +166,41 @@ def device_table(soap, a: np.ndarray, d: np.ndarray) -> torch.Tensor:
     return torch.from_numpy(tab.astype(np.float32)).to(soap.device)
 
 
+def contraction_fragments(soap, a: np.ndarray):
+    """fp16 operands of the contraction c[n][lm] = sum_pj a_lnpj Q_p[j][lm] on the matrix cores.
+
+    a: (slots, L1, n, INTERVALS, TERMS) or (L1, n, INTERVALS, TERMS).  Returns a float16 device
+    tensor (slots, L1, INTERVALS, 64 lanes, 16) holding, for lane = row + 16 g, the hi halves
+    then the lo halves of a_l[row][p][k] * 2^e_lrow at k = 4 g + (j % 4) + 16 (j / 4), j = 0..7
+    (the A operand of v_mfma_f32_16x16x32_f16, rows = n, contraction index matching the
+    accumulator lanes of Q), and the float32 inverse scales 2^-e_ln (slots, L1, 8).  e_ln puts
+    each row's largest coefficient in [2^13, 2^14)."""
+    S, L1, n = soap.n_species, soap._l_max + 1, soap._n_max
+    slots = (S + 1) * S
+    if a.ndim == 4:
+        a = np.broadcast_to(a, (slots,) + a.shape)
+    amax = np.abs(a).max(axis=(3, 4))                                       # (slots, L1, n)
+    e = np.where(amax > 0, 13 - np.floor(np.log2(np.where(amax > 0, amax, 1.0))), 0.0)
+    scaled = a * np.exp2(e)[..., None, None]
+    lane = np.arange(64)
+    row, g = lane % 16, lane // 16
+    j = np.arange(8)
+    k = 4 * g[:, None] + (j % 4)[None, :] + 16 * (j // 4)[None, :]           # (64, 8)
+    frag = np.zeros((slots, L1, INTERVALS, 64, 16), dtype=np.float16)
+    valid = row < n
+    rr = np.where(valid, row, 0)
+    for q in range(INTERVALS):
+        v = scaled[:, :, rr[:, None], q, k] * valid[:, None]                 # (slots, L1, 64, 8)
+        hi = v.astype(np.float16)
+        lo = (v - hi.astype(np.float64)).astype(np.float16)
+        frag[:, :, q, :, :8] = hi
+        frag[:, :, q, :, 8:] = lo
+    inv = np.zeros((slots, L1, N_PAD), dtype=np.float32)
+    inv[:, :, :n] = np.exp2(-e)
+    dev = soap.device
+    return torch.from_numpy(frag).to(dev), torch.from_numpy(inv).to(dev)
+
+
 def supported(soap) -> Optional[str]:
     """None when the matrix-core Chebyshev kernels cover this basis, else the reason."""
     if soap._rbf != "gto":

@@ -51,10 +51,95 @@ __host__ __device__ constexpr int lm_tiles(int lmax) { return ((lmax + 1) * (lma
 // the 16 rows of one operand read fall on different banks
 __device__ __forceinline__ int opnd(int row, int p) { return row * kCE + ((((p >> 3) ^ row) & 7) << 3) + (p & 7); }
 
+// Two values as fp16 hi + lo pairs (hi rounded to nearest, lo = fp16(v - hi)): ~22 bits each.
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void put_hilo2(_Float16* hi, _Float16* lo, int off0, int off1, float v0, float v1) {
+    const f2_t v = {v0, v1};
+    const h2_t h = __builtin_convertvector(v, h2_t);           // one packed conversion for both
+    const h2_t l = __builtin_convertvector(v - __builtin_convertvector(h, f2_t), h2_t);
+    hi[off0] = h.x;
+    hi[off1] = h.y;
+    lo[off0] = l.x;
+    lo[off1] = l.y;
+}
+
 __device__ __forceinline__ void put_hilo(_Float16* hi, _Float16* lo, int off, float v) {
     const _Float16 h = (_Float16)v;
     hi[off] = h;
     lo[off] = (_Float16)(v - (float)h);
+}
+
+// Real harmonics Y_lm(u) of the orders m of stripe `phase`, |u| = 1, the order pair (l, +m),
+// (l, -m) delivered together: put1(l, v) for m = 0, put2(l, m, v_+m, v_-m) for m > 0.  The
+// recurrence of solid_harmonics_stripe (associated Legendre in (z, r^2) with (x + i y)^m).
+// orders m split into four stripes of near-equal value counts (l_max + 1 values for m = 0,
+// 2 (l_max - m + 1) for m > 0), largest first into the lightest stripe: l_max 6 gives
+// {1}, {2, 6}, {3, 5}, {0, 4}
+template <int LMAX>
+struct StripeOf {
+    int s[LMAX + 1] = {};
+    constexpr StripeOf() {
+        int load[4] = {0, 0, 0, 0};
+        bool done[LMAX + 1] = {};
+        for (int it = 0; it <= LMAX; ++it) {
+            int best = -1, cnt = -1;
+            for (int m = 0; m <= LMAX; ++m) {
+                const int c = m == 0 ? LMAX + 1 : 2 * (LMAX - m + 1);
+                if (!done[m] && c > cnt) {
+                    cnt = c;
+                    best = m;
+                }
+            }
+            int t = 0;
+            for (int k = 1; k < 4; ++k)
+                if (load[k] < load[t]) t = k;
+            s[best] = t;
+            load[t] += cnt;
+            done[best] = true;
+        }
+    }
+};
+
+// As solid_harmonics_stripe, scaled by sc (folded into the start of the linear recurrence).
+template <int LMAX, typename P1, typename P2>
+__device__ __forceinline__ void harmonics_stripe_pairs(float x, float y, float z, int phase, float sc, P1&& put1,
+                                                       P2&& put2) {
+    constexpr StripeOf<LMAX> kStripe{};
+    const float r2 = (x * x + y * y) + z * z;
+    float cm = 1.0f, sm = 0.0f;
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) {
+        if (m > 0) {
+            const float nc = x * cm - y * sm;
+            const float ns = x * sm + y * cm;
+            cm = nc;
+            sm = ns;
+        }
+        if (kStripe.s[m] != phase) continue;
+        float q2 = 0.0f, q1 = (float)kDoubleFact[m] * sc;
+#pragma unroll
+        for (int l = m; l <= LMAX; ++l) {
+            float qv;
+            if (l == m) {
+                qv = q1;
+            } else if (l == m + 1) {
+                qv = (float)(2 * m + 1) * z * q1;
+            } else {
+                qv = ((float)(2 * l - 1) * z * q1 - (float)(l + m - 1) * r2 * q2) * (float)(1.0 / (l - m));
+            }
+            if (l > m) {
+                q2 = q1;
+                q1 = qv;
+            }
+            const float K = (float)kSHNorm[l][m];
+            if (m == 0)
+                put1(l, K * qv);
+            else
+                put2(l, m, K * qv * cm, K * qv * sm);
+        }
+    }
 }
 
 // 0xFFFF in half j of the four dwords where bit j of `bits` is set
@@ -71,9 +156,36 @@ __device__ __forceinline__ int degree_of(int lm, int lmax) {
     return l < lmax ? l : lmax;
 }
 
-// T_{8q} .. T_{8q+7} of t (q = 0..3): the start pair (T_{8q}, T_{8q+1}) from the product
-// rule T_{m+n} = 2 T_m T_n - T_{|m-n|}, then the three-term recurrence (fma)
-__device__ __forceinline__ void cheb_rows8(float t, int q, float (&T)[8]) {
+// Columns lm of the product packed into tiles of 16 by whole degrees (first fit, degrees
+// in decreasing order): l_max 6 gives {6, 1}, {5, 2}, {4, 3}, {0}, so a wavefront's
+// contraction touches at most two degrees.  row[t][i] = lm of column i of tile t, or -1.
+template <int LMAX>
+struct LmPack {
+    int ntile = 0;
+    int nl[4] = {0, 0, 0, 0};
+    int ls[4][4] = {};
+    int row[4][16] = {};
+    constexpr LmPack() {
+        int fill[4] = {0, 0, 0, 0};
+        for (int l = LMAX; l >= 0; --l) {
+            int t = 0;
+            while (fill[t] + 2 * l + 1 > 16) ++t;
+            if (t + 1 > ntile) ntile = t + 1;
+            ls[t][nl[t]++] = l;
+            fill[t] += 2 * l + 1;
+        }
+        for (int t = 0; t < 4; ++t) {
+            int i = 0;
+            for (int k = nl[t] - 1; k >= 0; --k)
+                for (int m = -ls[t][k]; m <= ls[t][k]; ++m) row[t][i++] = ls[t][k] * ls[t][k] + ls[t][k] + m;
+            for (; i < 16; ++i) row[t][i] = -1;
+        }
+    }
+};
+
+// sc T_{8q} .. sc T_{8q+7} of t (q = 0..3): the start pair (T_{8q}, T_{8q+1}) from the
+// product rule T_{m+n} = 2 T_m T_n - T_{|m-n|}, then the three-term recurrence (fma)
+__device__ __forceinline__ void cheb_rows8(float t, int q, float sc, float (&T)[8]) {
     const float t2 = fmaf(2.0f * t, t, -1.0f);
     float a = 1.0f, b = t;                                    // T_{8q}, T_{8q+1}
     if (q > 0) {
@@ -96,10 +208,21 @@ __device__ __forceinline__ void cheb_rows8(float t, int q, float (&T)[8]) {
             }
         }
     }
-    T[0] = a;
-    T[1] = b;
+    // the recurrence is linear: scaling the start pair scales every row
+    T[0] = a * sc;
+    T[1] = b * sc;
 #pragma unroll
     for (int k = 2; k < 8; ++k) T[k] = fmaf(2.0f * t, T[k - 1], -T[k - 2]);
+}
+
+// fp16 hi + lo of v (v - hi rounded again), as halves of two h8 operands
+__device__ __forceinline__ void split8(const float (&v)[8], h8_t& hi, h8_t& lo) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const _Float16 h = (_Float16)v[j];
+        hi[j] = h;
+        lo[j] = (_Float16)(v[j] - (float)h);
+    }
 }
 
 template <int LMAX, int SB>
@@ -111,14 +234,18 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
                                                          const int64_t* __restrict__ offsets,
                                                          const int32_t* __restrict__ nbr, int create_path, int n_centers,
                                                          int per_xcd, float* __restrict__ out) {
-    constexpr int L1 = LMAX + 1, L2 = L1 * L1, LT = lm_tiles(LMAX);
+    constexpr int L1 = LMAX + 1, L2 = L1 * L1;
+    constexpr LmPack<LMAX> kPack{};
+    constexpr int LT = kPack.ntile;
     constexpr int NG = 2 * SB;                                // (species, interval) groups
-    __shared__ __align__(16) _Float16 Th[kCT * kCE];
-    __shared__ __align__(16) _Float16 Tl[kCT * kCE];
-    __shared__ __align__(16) _Float16 Yh[(L2 + 1) * kCE];    // row L2: zeros (columns lm >= L2)
-    __shared__ __align__(16) _Float16 Yl[(L2 + 1) * kCE];
-    __shared__ __align__(16) float4 geo[4][kCE];              // (t, unit vector) by tile and position
-    __shared__ int gcnt[4][NG + 1];                           // edges per group, then the tile total
+    constexpr int TB = kCT * kCE, YB = (L2 + 1) * kCE;       // one operand buffer (halves)
+    constexpr int GCAP = 4 * kCE + 8 * NG;                    // sorted positions, groups 8-aligned
+    __shared__ __align__(16) _Float16 Th[2 * TB];             // two buffers: tile j and j + 1
+    __shared__ __align__(16) _Float16 Tl[2 * TB];
+    __shared__ __align__(16) _Float16 Yh[2 * YB];             // row L2 of a buffer: zeros
+    __shared__ __align__(16) _Float16 Yl[2 * YB];
+    __shared__ __align__(16) float4 geo[GCAP];                // (t, unit vector) by sorted position
+    __shared__ int wcnt[4][NG];                               // edges per wavefront and group
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, cr = lane & 15;
     const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront
     // consecutive workgroups go to different XCDs: give each XCD a contiguous range of
@@ -126,9 +253,10 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
     const int c = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (c >= n_centers) return;
     const int S = b.n_species, n = b.n_max;
-    if (tid < kCE) {
-        Yh[L2 * kCE + tid] = (_Float16)0.0f;
-        Yl[L2 * kCE + tid] = (_Float16)0.0f;
+    if (tid < 2 * kCE) {
+        const int bb = tid >> 6;
+        Yh[bb * YB + L2 * kCE + lane] = (_Float16)0.0f;
+        Yl[bb * YB + L2 * kCE + lane] = (_Float16)0.0f;
     }
     const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
     const int64_t e0 = offsets[c], e1 = offsets[c + 1];
@@ -140,20 +268,29 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
     // only build operands)
     const bool mine = q < LT;
     const int lt = q < LT ? q : 0;
+    const int lm_col = kPack.row[lt][cr];                     // this lane's column (-1: none)
+    const int yrow = lm_col >= 0 ? lm_col : L2;
+    // this lane's column offset in rows r = 0..7 (mod 8) of an operand block (the XOR swizzle)
+    int sw[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) sw[r] = ((((lane >> 3) ^ r) & 7) << 3) + (lane & 7);
 
     f4_t acc[NG][2];
 #pragma unroll
     for (int gi = 0; gi < NG; ++gi)
 #pragma unroll
         for (int kt = 0; kt < 2; ++kt) acc[gi][kt] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    int ctot[NG];                                             // the center's edges per group
+#pragma unroll
+    for (int gi = 0; gi < NG; ++gi) ctot[gi] = 0;
 
     for (int64_t r0 = e0; r0 < e1; r0 += 4 * kCE) {
-        // ---- geometry of a round of four tiles: wavefront q, tile q, lane = edge; the
-        // edge goes to position p of its tile, compacted by (species, interval)
+        // ---- geometry of a round of up to 256 edges (wavefront q, edges 64 q + lane),
+        // sorted by (species, interval) over the whole round
+        int grp = -1, rank = 0;
+        float4 gv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         {
             const int64_t e = r0 + q * kCE + lane;
-            int grp = -1;
-            float t = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
             if (e < e1) {
                 int img, a;
                 split_ext(nbr[e], n_atoms, inv_na, img, a);
@@ -166,106 +303,129 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
                     const float r = r2 * ir;
                     const int outer = r >= rs;
                     grp = 2 * atom_sp[a] + outer;
-                    t = outer ? fmaf(r - rs, tsc1, -1.0f) : fmaf(r, tsc0, -1.0f);
-                    ux = rx * ir;
-                    uy = ry * ir;
-                    uz = rz * ir;
+                    gv = make_float4(outer ? fmaf(r - rs, tsc1, -1.0f) : fmaf(r, tsc0, -1.0f), rx * ir, ry * ir,
+                                     rz * ir);
                 }
             }
-            int p = -1, nv = 0;
 #pragma unroll
             for (int gi = 0; gi < NG; ++gi) {
                 const uint64_t m = __ballot(grp == gi);
-                const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                if (grp == gi) p = nv + rank;
-                if (lane == 0) gcnt[q][gi] = __popcll(m);
-                nv += __popcll(m);
+                if (grp == gi)
+                    rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (lane == 0) wcnt[q][gi] = __popcll(m);
             }
-            if (lane == 0) gcnt[q][NG] = nv;
-            if (p >= 0) geo[q][p] = make_float4(t, ux, uy, uz);
         }
         __syncthreads();
-        const int ntile = (int)((e1 - r0 + kCE - 1) / kCE) < 4 ? (int)((e1 - r0 + kCE - 1) / kCE) : 4;
-        for (int j = 0; j < ntile; ++j) {
-            // ---- operands of tile j, lane = position: wavefront q writes the Chebyshev rows
-            // 8q .. 8q+7 and the harmonics of orders m = q (mod 4)
-            const int nv = gcnt[j][NG];
-            {
-                const float4 gv = geo[j][lane];
-                const bool ok = lane < nv;
-                const float sc = ok ? kOpScale : 0.0f;
-                const float t = ok ? gv.x : 0.0f;
-                float T[8];
-                cheb_rows8(t, q, T);
+        // group gi holds positions [gs, gs + gn), its start aligned to 8, so that every lane's
+        // 8 positions of an operand block belong to one group
+        int gs[NG], gn[NG], nv = 0;
+        {
+            int before = 0;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) put_hilo(Th, Tl, opnd(8 * q + k, lane), T[k] * sc);
-                solid_harmonics_stripe<LMAX, float>(ok ? gv.y : 0.0f, ok ? gv.z : 0.0f, ok ? gv.w : 0.0f, q,
-                                                    [&](int l, int m, float v) {
-                                                        put_hilo(Yh, Yl, opnd(l * l + l + m, lane), v * sc);
-                                                    });
+            for (int gi = 0; gi < NG; ++gi) {
+                const int c0 = wcnt[0][gi], c1 = wcnt[1][gi], c2 = wcnt[2][gi], c3 = wcnt[3][gi];
+                gs[gi] = nv;
+                gn[gi] = (c0 + c1) + (c2 + c3);
+                ctot[gi] += gn[gi];
+                if (grp == gi) before = nv + (q > 0 ? c0 : 0) + (q > 1 ? c1 : 0) + (q > 2 ? c2 : 0);
+                nv += (gn[gi] + 7) & ~7;
             }
-            __syncthreads();
-            // ---- products: wavefront q, columns lm = 16 q + lane % 16
-            if (mine) {
-                int beg[NG], cnt[NG];
-                {
-                    int acc_n = 0;
+            if (grp >= 0) geo[before + rank] = gv;
+        }
+        __syncthreads();
+        const int ntile = (nv + kCE - 1) / kCE;
+
+        // ---- operands of tile j into buffer j & 1, lane = position 64 j + lane: wavefront q
+        // writes the Chebyshev rows 8q .. 8q+7 and the harmonics of orders m = q (mod 4)
+        auto build = [&](int j) {
+            _Float16* th = Th + (j & 1) * TB;
+            _Float16* tl = Tl + (j & 1) * TB;
+            _Float16* yh = Yh + (j & 1) * YB;
+            _Float16* yl = Yl + (j & 1) * YB;
+            const int pp = kCE * j + lane;
+            bool ok = false;
 #pragma unroll
-                    for (int gi = 0; gi < NG; ++gi) {
-                        beg[gi] = acc_n;
-                        cnt[gi] = gcnt[j][gi];
-                        acc_n += cnt[gi];
-                    }
+            for (int gi = 0; gi < NG; ++gi) ok = ok || (pp >= gs[gi] && pp < gs[gi] + gn[gi]);
+            const float4 u = geo[ok ? pp : 0];
+            const float sc = ok ? kOpScale : 0.0f;
+            float T[8];
+            cheb_rows8(ok ? u.x : 0.0f, q, sc, T);
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) {
+                const int r0 = 8 * q + k;                     // rows r0, r0 + 1 (r0 = k mod 8)
+                put_hilo2(th, tl, r0 * kCE + sw[k], (r0 + 1) * kCE + sw[k + 1], T[k], T[k + 1]);
+            }
+            harmonics_stripe_pairs<LMAX>(
+                ok ? u.y : 0.0f, ok ? u.z : 0.0f, ok ? u.w : 0.0f, q, sc,
+                [&](int l, float v) {
+                    const int row = l * l + l;
+                    put_hilo(yh, yl, row * kCE + sw[row & 7], v);
+                },
+                [&](int l, int m, float vp, float vm) {
+                    const int rp = l * l + l + m, rm = l * l + l - m;
+                    put_hilo2(yh, yl, rp * kCE + sw[rp & 7], rm * kCE + sw[rm & 7], vp, vm);
+                });
+        };
+        // ---- products of tile j: wavefront q, columns lm = 16 q + lane % 16; load() issues the
+        // LDS reads of one 32-position block, mma() its products
+        struct Opnd {
+            uint4 ah[2], al[2];
+            h8_t bh, bl;
+        };
+        auto load = [&](int j, int blk, Opnd& o) {
+            const _Float16* th = Th + (j & 1) * TB;
+            const _Float16* tl = Tl + (j & 1) * TB;
+            const _Float16* yh = Yh + (j & 1) * YB;
+            const _Float16* yl = Yl + (j & 1) * YB;
+            const int ch = 4 * blk + g;
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt) {
+                const int row = 16 * kt + cr;
+                const int off = row * kCE + (((ch ^ row) & 7) << 3);
+                o.ah[kt] = *(const uint4*)(th + off);
+                o.al[kt] = *(const uint4*)(tl + off);
+            }
+            const int off = yrow * kCE + (((ch ^ yrow) & 7) << 3);
+            o.bh = *(const h8_t*)(yh + off);
+            o.bl = *(const h8_t*)(yl + off);
+        };
+        auto mma = [&](int j, int blk, const Opnd& o) {
+            const int b0 = kCE * j + 32 * blk, cs = b0 + 8 * g;   // this lane's 8 positions: [cs, cs + 8)
+            const h8_t x0 = __builtin_bit_cast(h8_t, o.ah[0]), x1 = __builtin_bit_cast(h8_t, o.ah[1]);
+            const h8_t y0 = __builtin_bit_cast(h8_t, o.al[0]), y1 = __builtin_bit_cast(h8_t, o.al[1]);
+#pragma unroll
+            for (int gi = 0; gi < NG; ++gi) {
+                const int lo = gs[gi], hi = gs[gi] + ((gn[gi] + 7) & ~7);
+                if (gn[gi] == 0 || hi <= b0 || lo >= b0 + 32) continue;
+                h8_t bh = o.bh, bl = o.bl;
+                if (!(lo <= b0 && hi >= b0 + 32)) {
+                    const bool in = cs >= lo && cs < hi;
+                    const h8_t z = h8_t{};
+                    bh = in ? bh : z;
+                    bl = in ? bl : z;
                 }
-                const int nb = (nv + 31) >> 5;
-                for (int blk = 0; blk < nb; ++blk) {
-                    const int ch = 4 * blk + g;
-                    uint4 ah[2], al[2];
-                    h8_t bh, bl;
-#pragma unroll
-                    for (int kt = 0; kt < 2; ++kt) {
-                        const int row = 16 * kt + cr;
-                        const int off = row * kCE + (((ch ^ row) & 7) << 3);
-                        ah[kt] = *(const uint4*)(Th + off);
-                        al[kt] = *(const uint4*)(Tl + off);
-                    }
-                    {
-                        int row = 16 * lt + cr;
-                        row = row < L2 ? row : L2;
-                        const int off = row * kCE + (((ch ^ row) & 7) << 3);
-                        bh = *(const h8_t*)(Yh + off);
-                        bl = *(const h8_t*)(Yl + off);
-                    }
-                    const int b0 = 32 * blk, p0 = b0 + 8 * g;
-#pragma unroll
-                    for (int gi = 0; gi < NG; ++gi) {
-                        const int lo = beg[gi], hi = beg[gi] + cnt[gi];
-                        if (cnt[gi] == 0 || hi <= b0 || lo >= b0 + 32) continue;
-                        uint4 mh[2], ml[2];
-                        if (lo <= b0 && hi >= b0 + 32) {
-#pragma unroll
-                            for (int kt = 0; kt < 2; ++kt) {
-                                mh[kt] = ah[kt];
-                                ml[kt] = al[kt];
-                            }
-                        } else {
-                            const int a0 = min(max(lo - p0, 0), 8), a1 = min(max(hi - p0, 0), 8);
-                            const uint32_t bits = ((1u << a1) - 1u) & ~((1u << a0) - 1u);
-                            const uint4 mk = half_mask(bits);
-#pragma unroll
-                            for (int kt = 0; kt < 2; ++kt) {
-                                mh[kt] = make_uint4(ah[kt].x & mk.x, ah[kt].y & mk.y, ah[kt].z & mk.z, ah[kt].w & mk.w);
-                                ml[kt] = make_uint4(al[kt].x & mk.x, al[kt].y & mk.y, al[kt].z & mk.z, al[kt].w & mk.w);
-                            }
-                        }
-#pragma unroll
-                        for (int kt = 0; kt < 2; ++kt) {
-                            const h8_t xh = __builtin_bit_cast(h8_t, mh[kt]), xl = __builtin_bit_cast(h8_t, ml[kt]);
-                            acc[gi][kt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bh, acc[gi][kt], 0, 0, 0);
-                            acc[gi][kt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, bl, acc[gi][kt], 0, 0, 0);
-                            acc[gi][kt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, bh, acc[gi][kt], 0, 0, 0);
-                        }
-                    }
+                acc[gi][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x0, bh, acc[gi][0], 0, 0, 0);
+                acc[gi][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x1, bh, acc[gi][1], 0, 0, 0);
+                acc[gi][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x0, bl, acc[gi][0], 0, 0, 0);
+                acc[gi][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x1, bl, acc[gi][1], 0, 0, 0);
+                acc[gi][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(y0, bh, acc[gi][0], 0, 0, 0);
+                acc[gi][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(y1, bh, acc[gi][1], 0, 0, 0);
+            }
+        };
+        build(0);
+        __syncthreads();
+        for (int j = 0; j < ntile; ++j) {
+            // the first block's reads of tile j are in flight while this wavefront writes the
+            // operands of tile j + 1 (other buffer); its products follow
+            const bool two = nv - kCE * j > 32;
+            Opnd o;
+            if (mine) load(j, 0, o);
+            if (j + 1 < ntile) build(j + 1);
+            if (mine) {
+                mma(j, 0, o);
+                if (two) {
+                    load(j, 1, o);
+                    mma(j, 1, o);
                 }
             }
             __syncthreads();
@@ -273,50 +433,68 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
     }
     if (!mine) return;
 
-    // c[s][n][lm] = 2^-28 sum_{p, j} a_lnpj Q_sp[j][lm]; this lane holds j = 16 kt + 4 g + i
-    const float* tab = b.rad_cheb;
-    const int lm = 16 * lt + cr;
-    const int dl = degree_of(lm, LMAX);
+    // ---- c[s][n][lm] = sum_{p,k} a_lnpk Q_sp[k][lm] on the matrix cores: A = the fp16 hi/lo
+    // coefficient rows (n) of each degree of this wavefront's tile, B = Q from the
+    // accumulators (this lane: column lm, k = 4 g + (j % 4) + 16 (j / 4)), columns of other
+    // degrees zeroed; B is scaled by 2^e_q (|Q| <= 1.03 x the group's edge count) before the
+    // hi/lo split.  All fragments of a species are loaded before its products.
+    const int dl = lm_col >= 0 ? degree_of(lm_col, LMAX) : -1;
+    constexpr int NLW = 4;                                    // degrees per tile, at most
+    const int nlw = kPack.nl[lt];
+    const h8_t* frag = (const h8_t*)b.cheb_frag;
 #pragma unroll
     for (int s = 0; s < SB; ++s) {
         if (s >= S) break;
-        const float* Al = tab + ((size_t)(cslot * S + s) * 2 * L1 + dl) * 2 * kCT * 8;
-        float cn[8];
+        const int slot = cslot * S + s;
+        h8_t fa[NLW][2][2];
 #pragma unroll
-        for (int nn = 0; nn < 8; ++nn) cn[nn] = 0.0f;
+        for (int li = 0; li < NLW; ++li) {
+            if (li >= nlw) break;
+            const int l = kPack.ls[lt][li];
 #pragma unroll
-        for (int pp = 0; pp < 2; ++pp)
-#pragma unroll
-            for (int kt = 0; kt < 2; ++kt) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int k = pp * kCT + 16 * kt + 4 * g + i;
-                    const float qv = acc[2 * s + pp][kt][i];
-                    const float4 a0 = *(const float4*)(Al + k * 8);
-                    const float4 a1 = *(const float4*)(Al + k * 8 + 4);
-                    cn[0] = fmaf(a0.x, qv, cn[0]);
-                    cn[1] = fmaf(a0.y, qv, cn[1]);
-                    cn[2] = fmaf(a0.z, qv, cn[2]);
-                    cn[3] = fmaf(a0.w, qv, cn[3]);
-                    cn[4] = fmaf(a1.x, qv, cn[4]);
-                    cn[5] = fmaf(a1.y, qv, cn[5]);
-                    cn[6] = fmaf(a1.z, qv, cn[6]);
-                    cn[7] = fmaf(a1.w, qv, cn[7]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
+            for (int pp = 0; pp < 2; ++pp) {
+                const h8_t* fr = frag + (((size_t)slot * L1 + l) * 2 + pp) * 64 * 2 + 2 * lane;
+                fa[li][pp][0] = fr[0];
+                fa[li][pp][1] = fr[1];
             }
-#pragma unroll
-        for (int nn = 0; nn < 8; ++nn) {
-            cn[nn] += __shfl_xor(cn[nn], 16, 64);
-            cn[nn] += __shfl_xor(cn[nn], 32, 64);
         }
-        if (lm < L2) {
+        const int nmax_g = max(ctot[2 * s], ctot[2 * s + 1]);
+        // 2^eq (1.03 nmax_g) <= 2^14
+        const int eq = 14 - (32 - __builtin_clz((uint32_t)(nmax_g + (nmax_g >> 5) + 1)));
+        const float qs = ldexpf(kOpUnscale, eq);
+        h8_t bh[2], bl[2];
 #pragma unroll
-            for (int nn = 0; nn < 8; ++nn) {
-                if ((nn >> 1) != g || nn >= n) continue;
-                float v = cn[nn] * kOpUnscale;
-                if (s == self_s && lm == 0) v += (float)b.self_l0[nn];
-                out[(((size_t)c * S + s) * n + nn) * L2 + lm] = v;
+        for (int pp = 0; pp < 2; ++pp) {
+            float v[8];
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) v[jj] = acc[2 * s + pp][jj >> 2][jj & 3] * qs;
+            split8(v, bh[pp], bl[pp]);
+        }
+        f4_t cacc = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int li = 0; li < NLW; ++li) {
+            if (li >= nlw) break;
+            const bool on = dl == kPack.ls[lt][li];
+            const h8_t z = h8_t{};
+#pragma unroll
+            for (int pp = 0; pp < 2; ++pp) {
+                const h8_t xh = on ? bh[pp] : z, xl = on ? bl[pp] : z;
+                cacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][0], xh, cacc, 0, 0, 0);
+                cacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][0], xl, cacc, 0, 0, 0);
+                cacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][1], xh, cacc, 0, 0, 0);
+            }
+        }
+        // this lane: rows n = 4 g + i of column lm
+        if (g < 2 && lm_col >= 0) {
+            const float* rinv = b.cheb_rowinv + ((size_t)slot * L1 + dl) * 8;
+            const float us = ldexpf(1.0f, -eq);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int nn = 4 * g + i;
+                if (nn >= n) continue;
+                float v = cacc[i] * rinv[nn] * us;
+                if (s == self_s && lm_col == 0) v += (float)b.self_l0[nn];
+                out[(((size_t)c * S + s) * n + nn) * L2 + lm_col] = v;
             }
         }
     }
@@ -380,6 +558,8 @@ int cheb_coefficients_impl(const cusoap_basis_t* b, const double* pos, int n_ato
                            const double* shifts, const double* centers, int n_centers, const int32_t* self_sp,
                            const int64_t* offsets, const int32_t* nbr, int create_path, float* out, hipStream_t st) {
     if (!self_sp) return fail(CUSOAP_ERR_ARG, "coefficients: the Chebyshev form needs center_self_species");
+    if (!b->cheb_frag || !b->cheb_rowinv)
+        return fail(CUSOAP_ERR_ARG, "coefficients: the Chebyshev form needs cheb_frag and cheb_rowinv");
     if (b->cheb_terms != kCT || !(b->cheb_split > 0.0) || !(b->cheb_split < b->cutoff))
         return fail(CUSOAP_ERR_UNSUPPORTED, "coefficients: the Chebyshev form needs cheb_terms 32 and 0 < cheb_split < cutoff");
     if (b->n_max > 8 || b->n_species > 4 || b->l_max > 6 || b->rbf != CUSOAP_RBF_GTO || b->weighting != CUSOAP_W_NONE)

@@ -352,6 +352,7 @@ class SOAP:
         self._cheb_terms = cheb.TERMS
         self._cheb_split = cheb.split_radius(self._cutoff)
         self._rad_cheb = cheb.device_table(self, a, d)
+        self._cheb_frag, self._cheb_rowinv = cheb.contraction_fragments(self, a)
         self._basis_learned = None
 
     # ---- basis (soap_generator.py:690-819) ---------------------------------
@@ -522,6 +523,8 @@ class SOAP:
                 bl.rad_cheb = self._rad_cheb.data_ptr()
                 bl.cheb_terms = self._cheb_terms
                 bl.cheb_split = self._cheb_split
+                bl.cheb_frag = self._cheb_frag.data_ptr()
+                bl.cheb_rowinv = self._cheb_rowinv.data_ptr()
             else:
                 bl.rad_table = self._rad_table.data_ptr()
                 bl.rad_intervals = self._rad_intervals

@@ -89,6 +89,12 @@ typedef struct cusoap_basis {
     const float* rad_cheb;
     int32_t cheb_terms;
     double cheb_split;
+    /* The same value coefficients as fp16 matrix-core operands: ((S+1) S slots, l_max+1,
+     * 2 intervals, 64 lanes, 16) halves, lane = n + 16 g holding 8 hi then 8 lo halves of
+     * a_l[n][k] 2^e_ln at k = 4 g + (j % 4) + 16 (j / 4), and cheb_rowinv ((S+1) S, l_max+1,
+     * 8) float32 = 2^-e_ln.  Required with rad_cheb. */
+    const void* cheb_frag;
+    const float* cheb_rowinv;
 } cusoap_basis_t;
 
 /* Cell-list geometry of one neighbour search. */
