@@ -215,14 +215,22 @@ __device__ __forceinline__ void cheb_rows8(float t, int q, float sc, float (&T)[
     for (int k = 2; k < 8; ++k) T[k] = fmaf(2.0f * t, T[k - 1], -T[k - 2]);
 }
 
-// fp16 hi + lo of v (v - hi rounded again), as halves of two h8 operands
+// fp16 hi + lo of v (v - hi rounded again) as two h8 operands, built from packed conversions
+// (two values per instruction, written straight into the operand dwords)
 __device__ __forceinline__ void split8(const float (&v)[8], h8_t& hi, h8_t& lo) {
+    union {
+        h2_t p[4];
+        h8_t v;
+    } H, Lo;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const _Float16 h = (_Float16)v[j];
-        hi[j] = h;
-        lo[j] = (_Float16)(v[j] - (float)h);
+    for (int j = 0; j < 4; ++j) {
+        const f2_t x = {v[2 * j], v[2 * j + 1]};
+        const h2_t h = __builtin_convertvector(x, h2_t);
+        H.p[j] = h;
+        Lo.p[j] = __builtin_convertvector(x - __builtin_convertvector(h, f2_t), h2_t);
     }
+    hi = H.v;
+    lo = Lo.v;
 }
 
 template <int LMAX, int SB>
@@ -233,7 +241,8 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
                                                          const int32_t* __restrict__ self_sp,
                                                          const int64_t* __restrict__ offsets,
                                                          const int32_t* __restrict__ nbr, int create_path, int n_centers,
-                                                         int per_xcd, float* __restrict__ out) {
+                                                         int per_xcd, double inv_na, float tsc0, float tsc1,
+                                                         float* __restrict__ out) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     constexpr LmPack<LMAX> kPack{};
     constexpr int LT = kPack.ntile;
@@ -262,8 +271,7 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
     const int64_t e0 = offsets[c], e1 = offsets[c + 1];
     const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
     const int cslot = self_sp[c] >= 0 ? self_sp[c] : S;
-    const double inv_na = 1.0 / (double)n_atoms;
-    const float rs = (float)b.cheb_split, tsc0 = 2.0f / rs, tsc1 = (float)(2.0 / (b.cutoff - b.cheb_split));
+    const float rs = (float)b.cheb_split;
     // wavefront q owns columns lm = 16 q .. 16 q + 15 of the product (waves q >= LT
     // only build operands)
     const bool mine = q < LT;
@@ -513,8 +521,12 @@ int launch_cheb_coeff(const cusoap_basis_t& b, const double* pos, int n_atoms, c
     } else {
         const int per = (n_centers + 7) / 8;
         prof_begin("coefficients", st);
+        // 1 / n_atoms (the extended-index split) and the interval scales 2 / width, once
+        const double inv_na = 1.0 / (double)n_atoms;
+        const float tsc0 = (float)(2.0 / b.cheb_split), tsc1 = (float)(2.0 / (b.cutoff - b.cheb_split));
         cheb_coeff_kernel<LMAX, SB><<<8 * per, 256, 0, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, self_sp,
-                                                             offsets, nbr, create_path, n_centers, per, out);
+                                                             offsets, nbr, create_path, n_centers, per, inv_na, tsc0,
+                                                             tsc1, out);
         prof_end("coefficients", st);
         CUSOAP_LAUNCH_CHECK();
         return CUSOAP_OK;
