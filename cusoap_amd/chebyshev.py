@@ -152,6 +152,35 @@ def fit_exact(soap):
     return a, d, ef, ed
 
 
+def fit_learned(soap, model, channels: np.ndarray):
+    """Per-slot coefficients (slots, L1, n, INTERVALS, TERMS) of the gated learned model: the
+    network's f = r_fit^l g_net(r) on [0, r_fit] for the channels (pair, l, n) in ``channels``,
+    the exact function elsewhere (other channels, radii beyond r_fit, slots of centers that
+    are not atoms).  The model jumps by its own error at r_fit; the series spreads that jump
+    over the outer interval at a fraction of its size."""
+    S, L1, n = soap.n_species, soap._l_max + 1, soap._n_max
+    slots = (S + 1) * S
+    pairs = channels.shape[0]
+    lv = np.arange(L1, dtype=np.float64)
+
+    def fun(r):
+        r = np.asarray(r, dtype=np.float64)
+        f_exact, _ = gto_projections(soap, r)                                   # (P, L1, n)
+        out = np.repeat(f_exact[:, None], slots, axis=1)                        # (P, slots, L1, n)
+        g, _ = model.evaluate(r)                                                # (pairs, P, O)
+        g = g.cpu().numpy().reshape(pairs, r.size, L1, n)
+        f_net = (model.r_fit ** lv)[None, None, :, None] * g
+        inside = r <= model.r_fit * (1.0 + 1e-12)
+        for pr in range(pairs):
+            m = np.asarray(channels[pr], dtype=bool)
+            sel = out[inside, pr]
+            sel[:, m] = f_net[pr, inside][:, m]
+            out[inside, pr] = sel
+        return out
+
+    return fit_function(fun, soap._cutoff)
+
+
 def device_table(soap, a: np.ndarray, d: np.ndarray) -> torch.Tensor:
     """float32 ((S+1) S, 2, L1, INTERVALS, TERMS, 8) device table: the same exact coefficients
     in every slot, or per-slot coefficients when a, d are (slots, L1, n, INTERVALS, TERMS)."""

@@ -293,7 +293,11 @@ class SOAP:
             self._rad_intervals = int(table_intervals)
             self._rad_table = radial_table(self, self._rad_intervals)
         elif radial == "chebyshev":
+            from .surrogate import radial_table
             self._init_chebyshev_radial()
+            # the adjoint edge pass reads the exact functions from the cubic-Hermite table
+            self._rad_intervals = int(table_intervals)
+            self._rad_table = radial_table(self, self._rad_intervals)
 
     def _sibling(self, **overrides) -> "SOAP":
         """A SOAP of the same basis (same Loewdin matrices) with some arguments replaced."""
@@ -329,6 +333,11 @@ class SOAP:
         exact = radial_table(self, n_intervals)
         net = radial_table(self, n_intervals, model=model, use_pair=self._learned_use_pair)
         self._rad_table = masked_table(exact, net, channels)
+        # the float coefficient stage reads the same gated model as Chebyshev series on the
+        # matrix cores (the adjoint edge pass keeps the table)
+        from . import chebyshev as cheb
+        if self.precision != "float64" and cheb.supported(self) is None:
+            self._init_chebyshev_radial(coefficients=cheb.fit_learned(self, model, channels))
 
     def _init_chebyshev_radial(self, coefficients=None):
         """Chebyshev series of the radial functions for the matrix-core kernels (exact
@@ -525,7 +534,7 @@ class SOAP:
                 bl.cheb_split = self._cheb_split
                 bl.cheb_frag = self._cheb_frag.data_ptr()
                 bl.cheb_rowinv = self._cheb_rowinv.data_ptr()
-            else:
+            if self._rad_table is not None:
                 bl.rad_table = self._rad_table.data_ptr()
                 bl.rad_intervals = self._rad_intervals
                 bl.rad_rmax = self._cutoff

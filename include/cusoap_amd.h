@@ -84,8 +84,9 @@ typedef struct cusoap_basis {
      * intervals, cheb_terms, 8): the coefficients a_j of f_nl(r) = r^l B_nl(r) (index 0)
      * and of df_nl/dr (index 1) in T_j(t), t = 2 (r - lo) / (hi - lo) - 1 on the interval
      * [lo, hi] holding r, n padded to 8, slots as for rad_table.  When set, the float
-     * coefficient and adjoint stages run on the matrix cores (fp16 hi/lo operands,
-     * float32 accumulation) and rad_table is not read. */
+     * coefficient stage (cusoap_coefficients with a float precision) runs on the matrix
+     * cores (fp16 hi/lo operands, float32 accumulation) and does not read rad_table; the
+     * adjoint edge pass reads rad_table (the same radial model tabulated). */
     const float* rad_cheb;
     int32_t cheb_terms;
     double cheb_split;
