@@ -242,7 +242,8 @@ def bench_adjoint(args):
     chosen = (args.precision, args.radial, args.spectrum)
     forms = [chosen]
     if not args.no_beside:
-        for f in (("float32", "exact", "exact"), ("float64", "exact", "exact"), ("mixed", "table", "exact")):
+        for f in (("float32", "exact", "exact"), ("float64", "exact", "exact"), ("mixed", "table", "exact"),
+                  ("mixed", "chebyshev", "exact")):
             if f not in forms:
                 forms.append(f)
     n_feat = soap_for(*chosen).get_number_of_features()

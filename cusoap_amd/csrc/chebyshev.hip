@@ -3,30 +3,37 @@
 // No reference counterpart for the arithmetic; the result is the coefficient array
 // of _coefficients_gto / _add_self_terms (soap_generator.py:1294-1391, 821-850) for
 // the radial functions that basis.rad_cheb represents.  Each radial function that
-// multiplies the unit-vector harmonics, f_nl(r) = r^l B_nl(r), is a Chebyshev series
-// sum_k a_lnk T_k(t), t = 2 r / cutoff - 1 (cusoap_amd/chebyshev.py).  Then
+// multiplies the unit-vector harmonics, f_nl(r) = r^l B_nl(r), is a piecewise Chebyshev
+// series on two intervals p, [0, r_s] and [r_s, cutoff] (r_s = basis.cheb_split), 32 terms
+// each: f_nl(r) = sum_k a_lnpk T_k(t_p(r)), t_p the interval's coordinate in [-1, 1]
+// (cusoap_amd/chebyshev.py).  Then
 //
-//   c[s][n][lm] = sum_{e of species s} f_nl(r_e) Y_lm(u_e) = sum_k a_lnk Q_s[k][lm],
-//   Q_s[k][lm]  = sum_{e of species s} T_k(t_e) Y_lm(u_e),
+//   c[s][n][lm] = sum_{e of species s} f_nl(r_e) Y_lm(u_e) = sum_{p,k} a_lnpk Q_sp[k][lm],
+//   Q_sp[k][lm] = sum_{e of species s in interval p} T_k(t_p(r_e)) Y_lm(u_e),
 //
-// and Q_s is a dense product over the edges with no l-dependence left in it: rows k,
+// and Q_sp is a dense product over the edges with no l-dependence left in it: rows k,
 // columns lm, contraction over edges.  It runs as v_mfma_f32_16x16x32_f16 products.
 //
-// One wavefront (workgroup) per center, tiles of 64 edges, lane = edge:
-//   1. geometry in float64, t and the unit vector in float32, T_k by the three-term
-//      recurrence (fma), the real harmonics Y_lm(u) (solid_harmonics at |u| = 1);
-//   2. both operands scaled by 2^14 and split into fp16 hi + lo (lo = fp16(v - hi)), so
-//      that hi*hi + hi*lo + lo*hi carries ~22 bits, and every value stays in the
-//      normal fp16 range or below 2^-14 absolute (2^-28 of the scaled maximum);
-//      stored to LDS rows [k][position] and [lm][position], 16-byte chunks of a row
-//      XOR-swizzled by the row, positions compacted by species (ballot + mbcnt);
-//   3. per 32 positions: A = T rows (k = 16 kt + lane % 16, positions 8 (lane / 16)..+7),
-//      B = Y rows (lm = 16 lt + lane % 16), three products per (kt, lt) tile into the
-//      float32 accumulators of each species present in the block; positions of other
-//      species are masked out of A, positions past the tile's edges hold zeros;
-//   4. after the last tile the accumulator lane (lm, k = 16 kt + 4 (lane / 16) + i)
-//      forms its partial sum_k a_lnk Q[k][lm] for 8 n, the four lane groups are added
-//      (xor 16, xor 32, fixed order) and the self term is added.
+// Four wavefronts (one workgroup) per center; consecutive workgroups go to different XCDs,
+// each XCD taking a contiguous range of centers.  Rounds of up to 256 edges:
+//   1. geometry (wavefront q: edges 64 q + lane) in float64, t and the unit vector in
+//      float32; the round's edges sorted by (species, interval) group (ballot + mbcnt),
+//      each group starting on a multiple of 8 positions;
+//   2. operands of 64-position tiles, double-buffered in LDS: wavefront q writes the
+//      Chebyshev rows 8 q .. 8 q + 7 (product-rule starts, then the recurrence) and the
+//      harmonics of the orders m in its stripe (StripeOf); both operands scaled by 2^14 and
+//      split into fp16 hi + lo (lo = fp16(v - hi)), so that hi*hi + hi*lo + lo*hi carries
+//      ~22 bits; rows [k][position] and [lm][position], 16-byte chunks XOR-swizzled by row;
+//   3. wavefront q owns lm tile q (LmPack: whole degrees packed into 16 columns) and forms
+//      Q for its columns, three products per (row tile, 32 positions) into the float32
+//      accumulators of each group present; a lane's 8 positions belong to one group, so
+//      positions of other groups are masked with one select; the products of tile j
+//      overlap the construction of tile j + 1;
+//   4. after the last round, c = sum_{p,k} a Q on the matrix cores: A = the precomputed
+//      fp16 hi/lo coefficient rows (n) of each degree of the tile (basis.cheb_frag,
+//      scaled per row, basis.cheb_rowinv undoing it), B = Q straight from the
+//      accumulators (their lanes are the B layout for k = 4 g + (j % 4) + 16 (j / 4)),
+//      columns of other degrees zeroed; then the self term.
 // Operand layouts of v_mfma_f32_16x16x32_f16: A[i][k] in lane i + 16 (k / 8), element
 // k % 8; B[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in lane j + 16 (i / 4),
 // element i % 4.  The contraction index only has to agree between A and B.
@@ -272,8 +279,8 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
     const int self_s = (create_path && self_sp) ? self_sp[c] : -1;
     const int cslot = self_sp[c] >= 0 ? self_sp[c] : S;
     const float rs = (float)b.cheb_split;
-    // wavefront q owns columns lm = 16 q .. 16 q + 15 of the product (waves q >= LT
-    // only build operands)
+    // wavefront q owns the columns of lm tile q of LmPack (waves q >= LT only build
+    // operands)
     const bool mine = q < LT;
     const int lt = q < LT ? q : 0;
     const int lm_col = kPack.row[lt][cr];                     // this lane's column (-1: none)

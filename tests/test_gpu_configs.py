@@ -185,7 +185,8 @@ _ATTACH_REF = {}
 
 
 @pytest.mark.parametrize("form", [("float64", "exact", "gto", 1e-10), ("mixed", "exact", "gto", 1e-5),
-                                  ("mixed", "table", "gto", 2e-5), ("float64", "exact", "polynomial", 1e-10)])
+                                  ("mixed", "table", "gto", 2e-5), ("mixed", "chebyshev", "gto", 2e-5),
+                                  ("float64", "exact", "polynomial", 1e-10)])
 @pytest.mark.parametrize("centers", [None, [5, [1.0, 2.0, 3.0], 17, 250, [0.5, 0.5, 0.5], 0]])
 def test_vjp_attach_against_oracle_jacobian(form, centers):
     prec, radial, rbf, tol = form
