@@ -172,6 +172,7 @@ struct LmPack {
     int nl[4] = {0, 0, 0, 0};
     int ls[4][4] = {};
     int row[4][16] = {};
+    int nlmax = 0;                                            // most degrees in one tile
     constexpr LmPack() {
         int fill[4] = {0, 0, 0, 0};
         for (int l = LMAX; l >= 0; --l) {
@@ -180,6 +181,7 @@ struct LmPack {
             if (t + 1 > ntile) ntile = t + 1;
             ls[t][nl[t]++] = l;
             fill[t] += 2 * l + 1;
+            if (nl[t] > nlmax) nlmax = nl[t];
         }
         for (int t = 0; t < 4; ++t) {
             int i = 0;
@@ -454,7 +456,7 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
     // degrees zeroed; B is scaled by 2^e_q (|Q| <= 1.03 x the group's edge count) before the
     // hi/lo split.  All fragments of a species are loaded before its products.
     const int dl = lm_col >= 0 ? degree_of(lm_col, LMAX) : -1;
-    constexpr int NLW = 4;                                    // degrees per tile, at most
+    constexpr int NLW = kPack.nlmax;                          // degrees per tile, at most
     const int nlw = kPack.nl[lt];
     const h8_t* frag = (const h8_t*)b.cheb_frag;
 #pragma unroll
