@@ -334,9 +334,13 @@ class SOAP:
         net = radial_table(self, n_intervals, model=model, use_pair=self._learned_use_pair)
         self._rad_table = masked_table(exact, net, channels)
         # the float coefficient stage reads the same gated model as Chebyshev series on the
-        # matrix cores (the adjoint edge pass keeps the table)
+        # matrix cores (the adjoint edge pass keeps the table) for one or two species; with
+        # more, the eight or more (species, interval) groups of a center each pad to whole
+        # operand blocks and the table kernel was measured faster (H/C/N/O box: 0.43 against
+        # 0.54 ms; water box 2.11 against 1.53 ms and the 1M cluster 17.2 against 13.1 ms
+        # the other way)
         from . import chebyshev as cheb
-        if self.precision != "float64" and cheb.supported(self) is None:
+        if self.precision != "float64" and cheb.supported(self) is None and self.n_species <= 2:
             self._init_chebyshev_radial(coefficients=cheb.fit_learned(self, model, channels))
 
     def _init_chebyshev_radial(self, coefficients=None):
