@@ -83,3 +83,42 @@ def test_supported_reasons():
     assert "l_max" in ch.supported(g)
     _, g = _soap(dict(r_cut=5.0, n_max=4, l_max=3, sigma=0.5, species=[1, 8]))
     assert ch.supported(g) is None
+
+
+def test_device_layouts_decode_to_the_coefficients():
+    """device_table and contraction_fragments hold the fitted coefficients in the layouts the
+    kernels read: the table as (slot, value/slope, l, interval, term, n padded to 8), the
+    fragments as hi + lo halves of a_l[n][p][k] 2^e_ln at lane n + 16 g, element j with
+    k = 4 g + (j % 4) + 16 (j / 4), and rowinv = 2^-e_ln putting each row maximum in
+    [2^13, 2^14)."""
+    p = BASES[0]
+    _, g = _soap(p)
+    a, d, _, _ = ch.fit_exact(g)
+    S, L1, n = g.n_species, p["l_max"] + 1, p["n_max"]
+    slots = (S + 1) * S
+    tab = ch.device_table(g, a, d).numpy()
+    assert tab.shape == (slots, 2, L1, ch.INTERVALS, ch.TERMS, ch.N_PAD)
+    for sl in (0, slots - 1):
+        assert np.abs(tab[sl, 0, :, :, :, :n] - np.moveaxis(a, 1, -1)).max() <= 1e-7 * np.abs(a).max()
+        assert np.abs(tab[sl, 1, :, :, :, :n] - np.moveaxis(d, 1, -1)).max() <= 1e-7 * np.abs(d).max()
+    assert not tab[..., n:].any()
+    frag, rowinv = ch.contraction_fragments(g, a)
+    frag, rowinv = frag.numpy().astype(np.float64), rowinv.numpy().astype(np.float64)
+    assert frag.shape == (slots, L1, ch.INTERVALS, 64, 16) and rowinv.shape == (slots, L1, ch.N_PAD)
+    lane = np.arange(64)
+    row, grp = lane % 16, lane // 16
+    j = np.arange(8)
+    k = 4 * grp[:, None] + (j % 4)[None, :] + 16 * (j // 4)[None, :]           # (64, 8)
+    for sl in (0, slots - 1):
+        for l in range(L1):
+            for q in range(ch.INTERVALS):
+                val = (frag[sl, l, q, :, :8] + frag[sl, l, q, :, 8:]) * 1.0     # hi + lo, scaled
+                for ln in range(64):
+                    if row[ln] >= n:
+                        assert not frag[sl, l, q, ln].any()
+                        continue
+                    ref = a[l, row[ln], q, k[ln]]
+                    got = val[ln] * rowinv[sl, l, row[ln]]
+                    assert np.abs(got - ref).max() <= 2.0 ** -21 * np.abs(a[l, row[ln]]).max() + 1e-30
+            rmax = np.abs(a[l, :, :, :]).max(axis=(1, 2)) / rowinv[sl, l, :n]    # scaled row maxima
+            assert np.all((rmax >= 2.0 ** 13) & (rmax < 2.0 ** 14))
