@@ -92,3 +92,22 @@ def test_chebyshev_four_species():
     ec = subset_err(g.create(s, centers=cen), o.create(s, centers=cen), ref_all)
     say(f"chebyshev H/C/N/O: desc {ed:.2e}, forces {ef:.2e}, mixed centers {ec:.2e}")
     assert ed < 2e-5 and ef < 2e-5 and ec < 2e-5
+
+
+def test_chebyshev_centers_with_several_rounds():
+    """r_cut 9 (cutoff 10.9 A): ~540 edges per center, so the kernel runs several 256-edge rounds per center
+    (groups accumulated across rounds, the contraction scale from the whole center's counts);
+    create and forces against the float64 oracle."""
+    s = systems.water_box(6, seed=6)
+    p = dict(r_cut=9.0, n_max=4, l_max=3, sigma=0.5, species=[1, 8], periodic=True)
+    o = so.OracleSOAP(**p)
+    g = SOAP(device=DEV, dtype="float64", precision="mixed", radial="chebyshev", _betas=o.betas, **p)
+    nl = g._neighbors(g._context(s, None), drop_self=False)
+    per_center = nl.n_edges / len(s["positions"])
+    w = np.random.default_rng(7).normal(size=o.get_number_of_features())
+    ref = o.create(s)
+    Fr = o.vjp(s, w[None, :].repeat(len(s["positions"]), 0))
+    d, F = g.descriptor_and_vjp(s, torch.tensor(w))
+    ed, ef = colerr(d, ref), np.abs(F.cpu().numpy() - Fr).max() / np.abs(Fr).max()
+    say(f"chebyshev r_cut 9: {per_center:.0f} edges per center (max {nl.max_count}), desc {ed:.2e}, forces {ef:.2e}")
+    assert nl.max_count > 2 * 256 and ed < 1e-5 and ef < 2e-5
