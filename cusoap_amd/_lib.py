@@ -57,8 +57,12 @@ _SIGS = {
     "cusoap_jacobian_dense": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i64, p, p, i32,
                               p, i32, p, sz, p, i32, i32, p],
     "cusoap_vjp_workspace_size": [C.POINTER(Basis), i32, C.POINTER(sz)],
+    "cusoap_vjp_ordered_workspace_size": [C.POINTER(Basis), i32, i32, i64, C.POINTER(sz)],
     "cusoap_vjp": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32, i32,
                    i32, p, sz, p, p],
+    "cusoap_edge_reverse": [p, i32, p, i32, p, i32, p, p, p, p],
+    "cusoap_vjp_ordered": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i64, p, i32, p, p, i32,
+                           i32, i32, p, sz, p, p],
     "cusoap_vjp_noalloc": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32,
                            i32, i32, p, p],
     "cusoap_lowdin_invsqrt_f64": [p, i32, i32, f64, p, p],

@@ -26,6 +26,16 @@
 // from the shared factors of each (l, |m|) pair (solid_harmonics_adjoint), and
 // the 3-vector is scattered with float64 atomics.  The polynomial basis uses a
 // float64 tile kernel.
+//
+// Fixed-order force sums (every atom its own center): the neighbour lists are then
+// symmetric, and rev[e] (edge_reverse_kernel, a binary search in the other atom's
+// range, which is sorted by extended index) is the position of the reverse edge.  The
+// edge pass stores each edge's contribution at its own index (coalesced), and
+// vjp_gather_kernel sums, for atom a, the contributions of the edges c -> a, found as
+// rev of the entries of a's own range, in float64 in a fixed order: the forces are the
+// same bits on every run.  An edge whose reverse is missing (a pair within rounding of
+// the cutoff, whose distance the two lists evaluate differently) adds its
+// contribution with a float64 atomic instead.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -38,7 +48,20 @@
 
 namespace cusoap {
 
+// Buffers of the fixed-order force sums (all null: atomics).  Edge indices are those of
+// offsets / nbr; e_max bounds them (the kernels also check offsets[n_centers] <= e_max).
+struct VjpOrder {
+    const int32_t* rev;   // (e_max) position of each edge's reverse edge, or -1 (atomic)
+    void* slots;          // (e_max, 3) each edge's contribution, float or double as the edge pass
+    double* att;          // (n_centers, 3) sum of each attached center's edge contributions
+    int64_t e_max;
+};
+
 namespace {
+
+__device__ __forceinline__ bool vjp_ordered(const VjpOrder& o, const int64_t* offsets, int n_centers) {
+    return o.rev != nullptr && offsets[n_centers] <= o.e_max;
+}
 
 __device__ __forceinline__ int lm_l2(int lm) {
     int l = (int)sqrtf((float)lm);
@@ -406,7 +429,7 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                                                       const int32_t* __restrict__ center_species,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr, const TM* __restrict__ M,
-                                                      int n_centers, int ngrp, double* __restrict__ grad) {
+                                                      int n_centers, int ngrp, double* __restrict__ grad, VjpOrder ord) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     constexpr int NT = 256;
     constexpr bool MIXED = RAD == kRadMixed;
@@ -426,12 +449,14 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     __shared__ int64_t goff[kVjpMaxGroup + 1];              // the group's edge offsets
     __shared__ double gcen[kVjpMaxGroup][3];
     __shared__ int gslot[kVjpMaxGroup];                        // table slot of each center
-    __shared__ double red[kVjpMaxGroup][3];                    // center-attach sums
+    __shared__ double red[NT / 64][kVjpMaxGroup][3];           // center-attach sums per wavefront
     constexpr bool HASH = FIXN && RAD == kRadTable;
     __shared__ int hkey[HASH ? kVjpHashSlots : 1];          // neighbour atom of each slot or -1
     __shared__ float hval[HASH ? kVjpHashSlots : 1][3];     // its summed contributions
-    const int tid = threadIdx.x, lane = tid & 63;
-    if (HASH)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const bool ordered = vjp_ordered(ord, offsets, n_centers);
+    TM* const slots = (TM*)ord.slots;
+    if (HASH && !ordered)
         for (int i = tid; i < kVjpHashSlots; i += NT) {
             hkey[i] = -1;
             hval[i][0] = hval[i][1] = hval[i][2] = 0.0f;
@@ -442,7 +467,7 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     if (tid < ng) {
         for (int d = 0; d < 3; ++d) {
             gcen[tid][d] = centers[3 * (size_t)(c0 + tid) + d];
-            red[tid][d] = 0.0;
+            for (int w = 0; w < NT / 64; ++w) red[w][tid][d] = 0.0;
         }
         gslot[tid] = (RAD == kRadTable && center_species[c0 + tid] >= 0) ? center_species[c0 + tid] : S;
     }
@@ -665,7 +690,13 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             gy = tryy * P + Y;
             gz = trz * P + Z;
             bool done = false;
-            if constexpr (HASH) {
+            if (ordered) {
+                // gathered through the reverse edge; without one, by atomic below
+                slots[3 * e] = gx;
+                slots[3 * e + 1] = gy;
+                slots[3 * e + 2] = gz;
+                done = ord.rev[e] >= 0;
+            } else if (HASH) {
                 unsigned h = ((unsigned)a * 2654435761u) >> 22;      // 10-bit multiplicative hash
                 for (int pr = 0; pr < kVjpHashProbe; ++pr, h = (h + 1) & (kVjpHashSlots - 1)) {
                     const int k = atomicCAS(&hkey[h], -1, a);
@@ -694,21 +725,27 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
 #pragma unroll
                     for (int d = 0; d < 3; ++d) v[d] += __shfl_xor(v[d], o, 64);
                 if (lane == 0)
-                    for (int d = 0; d < 3; ++d) atomicAdd(&red[cc][d], v[d]);
+                    for (int d = 0; d < 3; ++d) red[wv][cc][d] += v[d];   // this wavefront's own sums
             }
         }
     }
     if (HASH || center_attach) __syncthreads();
-    if (HASH)
+    if (HASH && !ordered)
         for (int i = tid; i < kVjpHashSlots; i += NT) {
             const int a = hkey[i];
             if (a >= 0)
                 for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)a + d], (double)hval[i][d]);
         }
     if (center_attach && tid < ng) {
-        const int ca = center_attach[c0 + tid];
-        if (ca >= 0)
-            for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)ca + d], -red[tid][d]);
+        const int c = c0 + tid, ca = center_attach[c];
+        if (ca >= 0) {
+            double sum[3];
+            for (int d = 0; d < 3; ++d) sum[d] = ((red[0][tid][d] + red[1][tid][d]) + red[2][tid][d]) + red[3][tid][d];
+            if (ordered && ca == c)
+                for (int d = 0; d < 3; ++d) ord.att[3 * (size_t)c + d] = sum[d];
+            else
+                for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)ca + d], -sum[d]);
+        }
     }
 }
 
@@ -721,9 +758,11 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
                                                       const int32_t* __restrict__ center_attach,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr, const double* __restrict__ M,
-                                                      double* __restrict__ grad) {
+                                                      int n_centers, double* __restrict__ grad, VjpOrder ord) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     const int n = b.n_max, S = b.n_species, NL = n * L1;
+    const bool ordered = vjp_ordered(ord, offsets, n_centers);
+    double* const slots = (double*)ord.slots;
     extern __shared__ __align__(16) unsigned char smem[];
     double* Ms = (double*)smem;               // [S][L2][n]
     double* R0 = Ms + S * n * L2;             // [64][NL]
@@ -799,9 +838,17 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
                 gy += ry * yh1 + yy * h0;
                 gz += rz * yh1 + yz * h0;
             });
-            atomicAdd(&grad[3 * (size_t)a], gx);
-            atomicAdd(&grad[3 * (size_t)a + 1], gy);
-            atomicAdd(&grad[3 * (size_t)a + 2], gz);
+            const int p = ordered ? ord.rev[e] : -1;
+            if (ordered) {                              // gathered through the reverse edge
+                slots[3 * e] = gx;
+                slots[3 * e + 1] = gy;
+                slots[3 * e + 2] = gz;
+            }
+            if (p < 0) {
+                atomicAdd(&grad[3 * (size_t)a], gx);
+                atomicAdd(&grad[3 * (size_t)a + 1], gy);
+                atomicAdd(&grad[3 * (size_t)a + 2], gz);
+            }
             sx += gx;
             sy += gy;
             sz += gz;
@@ -816,9 +863,15 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
             sz += __shfl_xor(sz, o, 64);
         }
         if (ca >= 0 && lane == 0) {
-            atomicAdd(&grad[3 * (size_t)ca], -sx);
-            atomicAdd(&grad[3 * (size_t)ca + 1], -sy);
-            atomicAdd(&grad[3 * (size_t)ca + 2], -sz);
+            if (ordered && ca == c) {
+                ord.att[3 * (size_t)c] = sx;
+                ord.att[3 * (size_t)c + 1] = sy;
+                ord.att[3 * (size_t)c + 2] = sz;
+            } else {
+                atomicAdd(&grad[3 * (size_t)ca], -sx);
+                atomicAdd(&grad[3 * (size_t)ca + 1], -sy);
+                atomicAdd(&grad[3 * (size_t)ca + 2], -sz);
+            }
         }
     }
 }
@@ -827,7 +880,7 @@ template <int LMAX, typename TM, int RAD, int NN>
 int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
                    const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
                    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
-                   double* grad, hipStream_t st) {
+                   double* grad, const VjpOrder& ord, hipStream_t st) {
     const size_t cols = sizeof(TM) * (NN > 0 ? 0 : 2 * 256 * (size_t)b.n_max) +
                         (RAD == kRadMixed && NN == 0 ? sizeof(double) * 256 * (size_t)b.n_max : 0);
     const size_t per_bytes = sizeof(TM) * (size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1);
@@ -844,7 +897,7 @@ int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, cons
     prof_begin("vjp_edges", st);
     kern<<<(n_centers + ngrp - 1) / ngrp, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach,
                                                            center_species, offsets, nbr, (const TM*)M, n_centers,
-                                                           ngrp, grad);
+                                                           ngrp, grad, ord);
     prof_end("vjp_edges", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -854,8 +907,8 @@ template <int LMAX>
 int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
                      const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
                      const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
-                     double* grad, hipStream_t st) {
-#define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, st
+                     double* grad, const VjpOrder& ord, hipStream_t st) {
+#define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, ord, st
     if (b.rad_table) {
         if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable, 0>(CUSOAP_VJP_ARGS);
         // registers for the radial vectors (float table form) at n_max 4..8
@@ -883,7 +936,7 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_edges", st);
     kern<<<n_centers, 64, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr,
-                                      (const double*)M, grad);
+                                      (const double*)M, n_centers, grad, ord);
     prof_end("vjp_edges", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -987,7 +1040,7 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
 #define CUSOAP_VJP_PARAMS const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp, \
     const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,                     \
     const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M, double* grad,          \
-    hipStream_t st
+    const VjpOrder& ord, hipStream_t st
 #define CUSOAP_VJP_DECL(L) int vjp_edges_l##L(CUSOAP_VJP_PARAMS);
 CUSOAP_VJP_DECL(0) CUSOAP_VJP_DECL(1) CUSOAP_VJP_DECL(2) CUSOAP_VJP_DECL(3) CUSOAP_VJP_DECL(4)
 CUSOAP_VJP_DECL(5) CUSOAP_VJP_DECL(6) CUSOAP_VJP_DECL(7) CUSOAP_VJP_DECL(8) CUSOAP_VJP_DECL(9)
@@ -997,7 +1050,7 @@ CUSOAP_VJP_DECL(5) CUSOAP_VJP_DECL(6) CUSOAP_VJP_DECL(7) CUSOAP_VJP_DECL(8) CUSO
 #define CUSOAP_VJP_DEF(L)                                                                                        \
     int vjp_edges_l##L(CUSOAP_VJP_PARAMS) {                                                                      \
         return launch_vjp_edges<L>(b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach,     \
-                                   center_species, offsets, nbr, M, grad, st);                                   \
+                                   center_species, offsets, nbr, M, grad, ord, st);                              \
     }
 #define CUSOAP_VJP_DEF2(L) CUSOAP_VJP_DEF(L)
 CUSOAP_VJP_DEF2(CUSOAP_ONLY_L)
@@ -1021,6 +1074,104 @@ int zero_f64(double* p, int64_t n, hipStream_t st) {
     return CUSOAP_OK;
 }
 
+__device__ __forceinline__ bool same_point(const double* __restrict__ u, const double* __restrict__ v) {
+    return u[0] == v[0] && u[1] == v[1] && u[2] == v[2];
+}
+
+__global__ void fill_rev_kernel(const int64_t* __restrict__ offsets, int n_centers, int32_t* __restrict__ rev) {
+    const int64_t n = offsets[n_centers];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        rev[i] = -1;
+}
+
+// rev for the edges of center c (one wavefront per center; consecutive workgroups go to
+// different XCDs, each XCD taking a contiguous range of centers, whose neighbours' ranges
+// then share its L2): the edge c -> a and its reverse a -> c (the image with the opposite
+// shift) are linked both ways by the one of the two with c < a (c == a: the image with the
+// lower index), which finds the other by binary search in a's range.  rev stays -1 (set
+// beforehand) when the reverse is missing or the lists are not the atoms' own
+// (centers[c] != positions[c] bitwise).  The image with the opposite shift mirrors img in
+// a -n..n grid of shifts; that guess is checked exactly, then every image is tried.
+__global__ __launch_bounds__(256) void edge_reverse_kernel(const double* __restrict__ pos, int n_atoms,
+                                                           const double* __restrict__ shifts, int n_img,
+                                                           const double* __restrict__ centers, int n_centers,
+                                                           int per_xcd, const int64_t* __restrict__ offsets,
+                                                           const int32_t* __restrict__ nbr, int32_t* __restrict__ rev) {
+    const int lane = threadIdx.x & 63;
+    const int c = ((int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3)) * 4 + (threadIdx.x >> 6);
+    if (c >= n_centers || c >= n_atoms) return;
+    if (!same_point(centers + 3 * (size_t)c, pos + 3 * (size_t)c)) return;
+    const double inv_na = 1.0 / (double)n_atoms;
+    for (int64_t e = offsets[c] + lane; e < offsets[c + 1]; e += 64) {
+        int img, a;
+        split_ext(nbr[e], n_atoms, inv_na, img, a);
+        if (a < c || a >= n_centers) continue;
+        const double* sh = shifts + 3 * (size_t)img;
+        auto opposite = [&](int t) {
+            return shifts[3 * t] == -sh[0] && shifts[3 * t + 1] == -sh[1] && shifts[3 * t + 2] == -sh[2];
+        };
+        int t = n_img - 1 - img;
+        if (!opposite(t)) {
+            t = -1;
+            for (int k = 0; k < n_img && t < 0; ++k)
+                if (opposite(k)) t = k;
+        }
+        if (t < 0 || (a == c && t > img)) continue;
+        if (a == c && t == img) {                  // its own reverse (the zero shift)
+            rev[e] = (int32_t)e;
+            continue;
+        }
+        if (!same_point(centers + 3 * (size_t)a, pos + 3 * (size_t)a)) continue;
+        const int key = t * n_atoms + c;
+        int64_t lo = offsets[a], hi = offsets[a + 1];
+        while (lo < hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            if (nbr[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < offsets[a + 1] && nbr[lo] == key) {
+            rev[e] = (int32_t)lo;
+            rev[lo] = (int32_t)e;
+        }
+    }
+}
+
+// grad[a] += the contributions of the edges c -> a (rev of the entries a -> c of a's own
+// range), summed in float64 in a fixed order (lane-strided, then a butterfly over the
+// wavefront), minus the sum of its own center's edges when that center moves with it.
+// One wavefront per atom.
+template <typename TM>
+__global__ __launch_bounds__(256) void vjp_gather_kernel(const int64_t* __restrict__ offsets, int n_centers,
+                                                         const int32_t* __restrict__ center_attach, VjpOrder ord,
+                                                         double* __restrict__ grad) {
+    if (!vjp_ordered(ord, offsets, n_centers)) return;
+    const int lane = threadIdx.x & 63;
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (a >= n_centers) return;
+    const TM* sl = (const TM*)ord.slots;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t p = offsets[a] + lane; p < offsets[a + 1]; p += 64) {
+        const int q = ord.rev[p];
+        if (q >= 0)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] += (double)sl[3 * (int64_t)q + d];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] += __shfl_xor(v[d], o, 64);
+    if (lane == 0) {
+        const bool moves = center_attach && center_attach[a] == a;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) grad[3 * (size_t)a + d] += moves ? v[d] - ord.att[3 * (size_t)a + d] : v[d];
+    }
+}
+
+// bytes of the fixed-order sums beyond the dL/dc rows: per center (attach sums) and per edge
+// (the three contributions in the edge pass's type)
+size_t order_center_bytes(int n_centers) { return align_up(sizeof(double) * 3 * (size_t)std::max(n_centers, 1), 256); }
+size_t order_edge_bytes(int prec) { return 3 * (prec == CUSOAP_PREC_F64 ? sizeof(double) : sizeof(float)); }
+
 // checks shared by both entry points; returns CUSOAP_OK or the error, and the precision
 // the kernels run in (the polynomial basis always runs in float64)
 int vjp_check(const cusoap_basis_t* b, int mode, int& prec, int n_centers, const int32_t* center_species,
@@ -1042,7 +1193,7 @@ int vjp_core(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
              const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
              const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs,
              int coeffs_dtype, const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast,
-             int n_feat, void* ws, double* grad, hipStream_t st) {
+             int n_feat, void* ws, double* grad, const VjpOrder& ord, hipStream_t st) {
     const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED && !b->rad_table) ? 1 : 0;
     const int fold_k = (b->rbf == CUSOAP_RBF_GTO || b->rad_table) ? 1 : 0;   // the GTO / table edge kernels
     int rc;
@@ -1059,7 +1210,7 @@ int vjp_core(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
 #undef CUSOAP_ADJ
     if (rc) return rc;
 #define CUSOAP_CASE(L) \
-    case L: return vjp_edges_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, grad, st);
+    case L: return vjp_edges_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, grad, ord, st);
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
@@ -1110,7 +1261,63 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
     if ((rc = zero_f64(grad, 3 * (int64_t)n_atoms, st))) return rc;
     if (n_centers == 0) return CUSOAP_OK;
     return vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
-                    offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad, st);
+                    offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad,
+                    VjpOrder{nullptr, nullptr, nullptr, 0}, st);
+}
+
+size_t vjp_ordered_ws_bytes(const cusoap_basis_t* b, int prec, int n_centers, int64_t n_edges) {
+    if (b->rbf != CUSOAP_RBF_GTO && !b->rad_table) prec = CUSOAP_PREC_F64;
+    const size_t e = (size_t)std::max<int64_t>(n_edges, 1);
+    return vjp_ws_bytes(b, n_centers) + order_center_bytes(n_centers) + align_up(order_edge_bytes(prec) * e, 256);
+}
+
+int edge_reverse_impl(const double* pos, int n_atoms, const double* shifts, int n_img, const double* centers,
+                      int n_centers, const int64_t* offsets, const int32_t* nbr, int32_t* rev, hipStream_t st) {
+    if (n_atoms < 1 || n_img < 1 || n_centers < 0) return fail(CUSOAP_ERR_ARG, "edge_reverse: bad sizes");
+    if (!pos || !shifts || !centers || !offsets || !nbr || !rev) return fail(CUSOAP_ERR_ARG, "edge_reverse: null argument");
+    if (n_centers == 0) return CUSOAP_OK;
+    // the edge count is read on the device (offsets[n_centers]): -1 over the whole range
+    // by a grid-stride fill bounded by the last offset
+    prof_begin("edge_reverse", st);
+    fill_rev_kernel<<<1024, 256, 0, st>>>(offsets, n_centers, rev);
+    CUSOAP_LAUNCH_CHECK();
+    const int groups = (n_centers + 3) / 4, per = (groups + 7) / 8;
+    edge_reverse_kernel<<<8 * per, 256, 0, st>>>(pos, n_atoms, shifts, n_img, centers, n_centers, per, offsets, nbr,
+                                                 rev);
+    prof_end("edge_reverse", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
+int vjp_ordered_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
+                     const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
+                     const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const int32_t* rev,
+                     int64_t n_edges, const void* coeffs, int coeffs_dtype, const int32_t* pair_feature, const void* gd,
+                     int gd_dtype, int gd_broadcast, int n_feat, void* ws, size_t ws_bytes, double* grad, hipStream_t st) {
+    (void)n_img;
+    int rc = vjp_check(b, mode, prec, n_centers, center_species, pair_feature);
+    if (rc) return rc;
+    if (n_centers != n_atoms) return fail(CUSOAP_ERR_ARG, "vjp_ordered: every atom must be its own center");
+    if (!rev) return fail(CUSOAP_ERR_ARG, "vjp_ordered: reverse-edge index missing");
+    if (n_edges < 0 || n_edges > INT32_MAX) return fail(CUSOAP_ERR_ARG, "vjp_ordered: edge count out of range");
+    const size_t base = vjp_ws_bytes(b, n_centers);
+    if (ws_bytes < vjp_ordered_ws_bytes(b, prec, n_centers, n_edges))
+        return fail(CUSOAP_ERR_WORKSPACE, "vjp_ordered: workspace too small");
+    if ((rc = zero_f64(grad, 3 * (int64_t)n_atoms, st))) return rc;
+    if (n_centers == 0) return CUSOAP_OK;
+    char* p = (char*)ws + base;
+    const VjpOrder ord{rev, p + order_center_bytes(n_centers), (double*)p, n_edges};
+    rc = vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
+                  offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad, ord, st);
+    if (rc) return rc;
+    prof_begin("vjp_gather", st);
+    if (prec == CUSOAP_PREC_F64)
+        vjp_gather_kernel<double><<<(n_centers + 3) / 4, 256, 0, st>>>(offsets, n_centers, center_attach, ord, grad);
+    else
+        vjp_gather_kernel<float><<<(n_centers + 3) / 4, 256, 0, st>>>(offsets, n_centers, center_attach, ord, grad);
+    prof_end("vjp_gather", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
 }
 
 // Centers in chunks whose workspace fits the static array; the chunks add into grad.
@@ -1144,7 +1351,7 @@ int vjp_noalloc_impl(const cusoap_basis_t* b, int mode, int prec, const double* 
                       center_attach ? center_attach + c0 : nullptr, center_species ? center_species + c0 : nullptr,
                       offsets + c0, nbr, (const char*)coeffs + csz * per * c0, coeffs_dtype, pair_feature,
                       gd_broadcast ? gd : (const void*)((const char*)gd + gsz * (size_t)n_feat * c0), gd_dtype,
-                      gd_broadcast, n_feat, ws, grad, st);
+                      gd_broadcast, n_feat, ws, grad, VjpOrder{nullptr, nullptr, nullptr, 0}, st);
         if (rc) return rc;
     }
     return CUSOAP_OK;

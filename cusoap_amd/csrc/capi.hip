@@ -71,6 +71,13 @@ int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const in
                         const double*, int, const int32_t*, const int64_t*, const int32_t*, int64_t, const double*,
                         const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, int, hipStream_t);
 size_t vjp_ws_bytes(const cusoap_basis_t*, int);
+size_t vjp_ordered_ws_bytes(const cusoap_basis_t*, int, int, int64_t);
+int edge_reverse_impl(const double*, int, const double*, int, const double*, int, const int64_t*, const int32_t*,
+                      int32_t*, hipStream_t);
+int vjp_ordered_impl(const cusoap_basis_t*, int, int, const double*, int, const int32_t*, const double*, int,
+                     const double*, int, const int32_t*, const int32_t*, const int64_t*, const int32_t*,
+                     const int32_t*, int64_t, const void*, int, const int32_t*, const void*, int, int, int, void*,
+                     size_t, double*, hipStream_t);
 int vjp_impl(const cusoap_basis_t*, int, int, const double*, int, const int32_t*, const double*, int, const double*,
              int, const int32_t*, const int32_t*, const int64_t*, const int32_t*, const void*, int, const int32_t*,
              const void*, int, int, int, void*, size_t, double*, hipStream_t);
@@ -201,6 +208,15 @@ int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, si
     return CUSOAP_OK;
 }
 
+int cusoap_vjp_ordered_workspace_size(const cusoap_basis_t* basis, int32_t precision, int32_t n_centers,
+                                      int64_t n_edges, size_t* bytes) {
+    if (!basis || !bytes) return fail(CUSOAP_ERR_ARG, "vjp_ordered_workspace_size: null argument");
+    if (precision < 0 || precision > 2) return fail(CUSOAP_ERR_ARG, "vjp_ordered_workspace_size: unknown precision");
+    if (n_centers < 0 || n_edges < 0) return fail(CUSOAP_ERR_ARG, "vjp_ordered_workspace_size: negative size");
+    *bytes = vjp_ordered_ws_bytes(basis, precision, n_centers, n_edges);
+    return CUSOAP_OK;
+}
+
 int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision, const double* positions,
                int32_t n_atoms, const int32_t* atom_species, const double* shifts, int32_t n_img,
                const double* centers, int32_t n_centers, const int32_t* center_attach,
@@ -210,6 +226,26 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision, con
     return vjp_impl(basis, mode, precision, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
                     center_attach, center_species, offsets, nbr, coeffs, coeffs_dtype, pair_feature, grad_desc,
                     gd_dtype, gd_broadcast, n_feat, ws, ws_bytes, grad_pos, (hipStream_t)stream);
+}
+
+int cusoap_edge_reverse(const double* positions, int32_t n_atoms, const double* shifts, int32_t n_img,
+                        const double* centers, int32_t n_centers, const int64_t* offsets, const int32_t* nbr,
+                        int32_t* rev, void* stream) {
+    return edge_reverse_impl(positions, n_atoms, shifts, n_img, centers, n_centers, offsets, nbr, rev,
+                             (hipStream_t)stream);
+}
+
+int cusoap_vjp_ordered(const cusoap_basis_t* basis, int32_t mode, int32_t precision, const double* positions,
+                       int32_t n_atoms, const int32_t* atom_species, const double* shifts, int32_t n_img,
+                       const double* centers, int32_t n_centers, const int32_t* center_attach,
+                       const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const int32_t* rev,
+                       int64_t n_edges, const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
+                       const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat, void* ws,
+                       size_t ws_bytes, double* grad_pos, void* stream) {
+    return vjp_ordered_impl(basis, mode, precision, positions, n_atoms, atom_species, shifts, n_img, centers, n_centers,
+                            center_attach, center_species, offsets, nbr, rev, n_edges, coeffs, coeffs_dtype,
+                            pair_feature, grad_desc, gd_dtype, gd_broadcast, n_feat, ws, ws_bytes, grad_pos,
+                            (hipStream_t)stream);
 }
 
 int cusoap_vjp_noalloc(const cusoap_basis_t* basis, int32_t mode, int32_t precision, const double* positions,

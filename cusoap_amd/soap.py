@@ -280,6 +280,11 @@ class SOAP:
             raise ValueError(f"spectrum must be one of {sorted(_SPECTRA)}")
         self.spectrum = spectrum
         self.radial = radial
+        # True: fixed-order force sums in the adjoint when every atom is a center (forces
+        # the same bits on every run; cusoap_edge_reverse + cusoap_vjp_ordered, ~0.7 ms more
+        # on a 98,304-atom water box step); False (default): float64 atomics
+        self.ordered_forces = False
+        self._side_stream = None
         self.surrogate = None
         self._rad_table = None
         self._rad_cheb = None
@@ -1131,7 +1136,32 @@ class SOAP:
         closed = (not self.periodic) and self._weighting is None
         return self._coefficients(x, nl_keep, create_path=not closed)
 
-    def _vjp_ctx(self, x: _Ctx, grad_desc, centers, attach, coeffs=None, nl_keep=None):
+    def _edge_reverse(self, x: _Ctx, nl: _NL):
+        """(rev, event) for the fixed-order force sums, or None when they do not apply.
+
+        rev[e] is the position of the reverse edge of e (cusoap_edge_reverse).  It depends on
+        the neighbour lists only, so it runs on a second stream (default priority) while the
+        coefficients are formed; the adjoint waits on the event.
+        """
+        if not (self.ordered_forces and x.n_centers == x.n_atoms and x.n_centers > 0 and nl.n_edges > 0):
+            return None
+        lib = _lib.load()
+        main = torch.cuda.current_stream(self.device)
+        if self._side_stream is None:
+            self._side_stream = torch.cuda.Stream(self.device)
+        side = self._side_stream
+        side.wait_stream(main)
+        rev = torch.empty(nl.n_edges, dtype=torch.int32, device=self.device)
+        _lib.check(lib.cusoap_edge_reverse(x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
+                                           x.centers.data_ptr(), x.n_centers, nl.offsets.data_ptr(),
+                                           nl.nbr.data_ptr(), rev.data_ptr(), side.cuda_stream), "edge_reverse")
+        for t in (rev, x.pos, x.shifts, x.centers, nl.offsets, nl.nbr):
+            t.record_stream(side)
+        ev = torch.cuda.Event()
+        ev.record(side)
+        return rev, ev
+
+    def _vjp_ctx(self, x: _Ctx, grad_desc, centers, attach, coeffs=None, nl_keep=None, ordered=True, rev=None):
         lib = _lib.load()
         b = self._basis()
         n_feat = self.get_number_of_features()
@@ -1150,31 +1180,45 @@ class SOAP:
             bcast = 0
         if nl_keep is None:
             nl_keep = self._neighbors(x, drop_self=False)
+        if rev is None and ordered:
+            rev = self._edge_reverse(x, nl_keep)
         if coeffs is None:
             coeffs = self._vjp_coefficients(x, nl_keep)
-        nbytes = C.c_size_t(0)
-        _lib.check(lib.cusoap_vjp_workspace_size(C.byref(b), x.n_centers, C.byref(nbytes)), "vjp_workspace_size")
-        ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
         grad = torch.empty((x.n_atoms, 3), dtype=torch.float64, device=self.device)
         att = self._attach_array(x, centers, attach)
         mode = _lib.MODE_CC if self.average == "cc" else _lib.MODE_PS
-        _lib.check(lib.cusoap_vjp(
-            C.byref(b), mode, _PRECISIONS[self.precision], x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(),
-            x.shifts.data_ptr(), x.n_img, x.centers.data_ptr(), x.n_centers,
-            att.data_ptr() if att is not None else None, x.self_sp.data_ptr(), nl_keep.offsets.data_ptr(),
-            nl_keep.nbr.data_ptr(),
-            coeffs.data_ptr(), _lib.DT_F64 if coeffs.dtype == torch.float64 else _lib.DT_F32,
-            self._keep["pf"].data_ptr(), g.data_ptr(), _lib.DT_F64 if g.dtype == torch.float64 else _lib.DT_F32,
-            bcast, n_feat, ws.data_ptr(), ws.numel(), grad.data_ptr(), self._stream()), "vjp")
+        prec = _PRECISIONS[self.precision]
+        nbytes = C.c_size_t(0)
+        head = (C.byref(b), mode, prec, x.pos.data_ptr(), x.n_atoms, x.sp.data_ptr(),
+                x.shifts.data_ptr(), x.n_img, x.centers.data_ptr(), x.n_centers,
+                att.data_ptr() if att is not None else None, x.self_sp.data_ptr(), nl_keep.offsets.data_ptr(),
+                nl_keep.nbr.data_ptr())
+        tail = (coeffs.data_ptr(), _lib.DT_F64 if coeffs.dtype == torch.float64 else _lib.DT_F32,
+                self._keep["pf"].data_ptr(), g.data_ptr(), _lib.DT_F64 if g.dtype == torch.float64 else _lib.DT_F32,
+                bcast, n_feat)
+        if ordered and rev is not None:
+            rev_t, ev = rev
+            torch.cuda.current_stream(self.device).wait_event(ev)
+            _lib.check(lib.cusoap_vjp_ordered_workspace_size(C.byref(b), prec, x.n_centers, nl_keep.n_edges,
+                                                             C.byref(nbytes)), "vjp_ordered_workspace_size")
+            ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
+            _lib.check(lib.cusoap_vjp_ordered(*head, rev_t.data_ptr(), nl_keep.n_edges, *tail, ws.data_ptr(),
+                                              ws.numel(), grad.data_ptr(), self._stream()), "vjp_ordered")
+        else:
+            _lib.check(lib.cusoap_vjp_workspace_size(C.byref(b), x.n_centers, C.byref(nbytes)), "vjp_workspace_size")
+            ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
+            _lib.check(lib.cusoap_vjp(*head, *tail, ws.data_ptr(), ws.numel(), grad.data_ptr(), self._stream()),
+                       "vjp")
         return grad
 
     def descriptor_and_vjp(self, system, grad_desc, centers=None, attach=False):
         """Descriptor and its adjoint from one neighbour search and one coefficient pass."""
         x = self._context(system, centers)
         nl_keep = self._neighbors(x, drop_self=False)
+        rev = self._edge_reverse(x, nl_keep)
         coeffs = self._vjp_coefficients(x, nl_keep)
         desc = self._descriptor(coeffs)
-        return desc, self._vjp_ctx(x, grad_desc, centers, attach, coeffs=coeffs, nl_keep=nl_keep)
+        return desc, self._vjp_ctx(x, grad_desc, centers, attach, coeffs=coeffs, nl_keep=nl_keep, rev=rev)
 
     def create_autograd(self, positions: torch.Tensor, atomic_numbers, cell=None, pbc=None, centers=None,
                         attach=False) -> torch.Tensor:

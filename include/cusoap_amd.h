@@ -229,7 +229,9 @@ int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
  * index of each channel pair or -1 (power spectrum only).  center_attach as
  * for the Jacobian.  center_species: (n_centers) species of the center atom or
  * -1, required when basis->rad_table is set (table slot), else may be NULL.
- * Forces are accumulated with float64 atomics. */
+ * Forces are accumulated with float64 atomics (the summation order, and so the
+ * last bits, vary from run to run); cusoap_vjp_ordered below sums in a fixed
+ * order. */
 int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes);
 int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
                const double* positions, int32_t n_atoms, const int32_t* atom_species,
@@ -239,6 +241,37 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
                const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
                const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
                void* ws, size_t ws_bytes, double* grad_pos, void* stream);
+
+/* ---- fixed-order force sums (no reference counterpart: the reference's
+ *      index_add_ scatters, soap_generator.py:1375-1381, 2422, are atomic and
+ *      run-to-run variable on the GPU) -------------------------------------------
+ * cusoap_edge_reverse: rev (offsets[n_centers]) int32, for each edge c -> a the
+ * position of the reverse edge a -> c (image with the opposite shift) in atom
+ * a's range of the same lists, or -1 when it is missing (a pair within rounding
+ * of the cutoff) or the lists are not the atoms' own (centers[c] or centers[a]
+ * differs from the atom's position, bitwise).  Depends on the neighbour lists
+ * only, so it may run on another stream while the coefficients are formed.
+ * cusoap_vjp_ordered: cusoap_vjp for every atom its own center (n_centers ==
+ * n_atoms, centers[c] == positions[c]) with rev from cusoap_edge_reverse and
+ * n_edges = offsets[n_centers]: each edge's contribution is stored at its own
+ * index, and each atom's force is the float64 sum, in a fixed order, of the
+ * contributions of its reverse edges, so the result is the same bits on every
+ * run.  Edges with rev = -1 add atomically (as cusoap_vjp).  Workspace:
+ * cusoap_vjp_ordered_workspace_size bytes. */
+int cusoap_edge_reverse(const double* positions, int32_t n_atoms, const double* shifts, int32_t n_img,
+                        const double* centers, int32_t n_centers, const int64_t* offsets, const int32_t* nbr,
+                        int32_t* rev, void* stream);
+int cusoap_vjp_ordered_workspace_size(const cusoap_basis_t* basis, int32_t precision, int32_t n_centers,
+                                      int64_t n_edges, size_t* bytes);
+int cusoap_vjp_ordered(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
+                       const double* positions, int32_t n_atoms, const int32_t* atom_species,
+                       const double* shifts, int32_t n_img,
+                       const double* centers, int32_t n_centers, const int32_t* center_attach,
+                       const int32_t* center_species, const int64_t* offsets, const int32_t* nbr,
+                       const int32_t* rev, int64_t n_edges,
+                       const void* coeffs, int32_t coeffs_dtype, const int32_t* pair_feature,
+                       const void* grad_desc, int32_t gd_dtype, int32_t gd_broadcast, int32_t n_feat,
+                       void* ws, size_t ws_bytes, double* grad_pos, void* stream);
 
 /* Allocation-free cusoap_vjp.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving or MD loops,
