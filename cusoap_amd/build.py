@@ -36,13 +36,20 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build cusoap_amd)")
 
 
+# per-source flags: the Chebyshev coefficient kernel keeps its matrix-core accumulators in
+# ordinary VGPRs (no accumulator-register copies around the hi/lo split of the epilogue;
+# same register total and occupancy)
+SOURCE_FLAGS = {"chebyshev.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def units():
     """(source, extra flags, object name) of every compilation."""
     out = []
     for src in SOURCES:
-        out.append((src, [], src.replace(".hip", ".o")))
+        extra = SOURCE_FLAGS.get(src, [])
+        out.append((src, list(extra), src.replace(".hip", ".o")))
         if src in PER_DEGREE:
-            out += [(src, [f"-DCUSOAP_ONLY_L={l}"], src.replace(".hip", f"_l{l}.o")) for l in range(MAX_L + 1)]
+            out += [(src, [*extra, f"-DCUSOAP_ONLY_L={l}"], src.replace(".hip", f"_l{l}.o")) for l in range(MAX_L + 1)]
     return out
 
 
@@ -61,7 +68,9 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     todo = units()
     jobs = jobs or min(len(todo), os.cpu_count() or 4, 16)
     # the heaviest compilations first, so the pool ends together
-    todo.sort(key=lambda u: (u[0] not in PER_DEGREE, -int(u[1][0].split("=")[1]) if u[1] else 0))
+    def degree(u):
+        return max([int(f.split("=")[1]) for f in u[1] if f.startswith("-DCUSOAP_ONLY_L=")] or [0])
+    todo.sort(key=lambda u: (u[0] not in PER_DEGREE, -degree(u)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, todo))
     tmp = LIB + ".tmp"

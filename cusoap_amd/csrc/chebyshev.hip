@@ -487,19 +487,20 @@ __global__ __launch_bounds__(256) void cheb_coeff_kernel(cusoap_basis_t b, const
             for (int jj = 0; jj < 8; ++jj) v[jj] = acc[2 * s + pp][jj >> 2][jj & 3] * qs;
             split8(v, bh[pp], bl[pp]);
         }
+        // one accumulator per degree of the tile (every column, rows n of that degree's
+        // coefficients); each lane keeps the one of its column's degree
         f4_t cacc = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int li = 0; li < NLW; ++li) {
             if (li >= nlw) break;
-            const bool on = dl == kPack.ls[lt][li];
-            const h8_t z = h8_t{};
+            f4_t dacc = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int pp = 0; pp < 2; ++pp) {
-                const h8_t xh = on ? bh[pp] : z, xl = on ? bl[pp] : z;
-                cacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][0], xh, cacc, 0, 0, 0);
-                cacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][0], xl, cacc, 0, 0, 0);
-                cacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][1], xh, cacc, 0, 0, 0);
+                dacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][0], bh[pp], dacc, 0, 0, 0);
+                dacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][0], bl[pp], dacc, 0, 0, 0);
+                dacc = __builtin_amdgcn_mfma_f32_16x16x32_f16(fa[li][pp][1], bh[pp], dacc, 0, 0, 0);
             }
+            if (dl == kPack.ls[lt][li]) cacc = dacc;
         }
         // this lane: rows n = 4 g + i of column lm
         if (g < 2 && lm_col >= 0) {
