@@ -248,8 +248,8 @@ def bench_adjoint(args):
                 forms.append(f)
     n_feat = soap_for(*chosen).get_number_of_features()
     w = torch.tensor(readout_weights(n_feat), device=dev)
-    names = ("coefficients", "vjp_edges", "vjp_adjoint", "edge_reverse", "vjp_gather", "nl_count", "nl_fill",
-             "power_spectrum")
+    names = ("coefficients", "vjp_edges", "vjp_adjoint", "edge_reverse", "vjp_gather", "center_order", "nl_count",
+             "nl_fill", "power_spectrum")
 
     def time_form(form):
         sp = soap_for(*form)
@@ -301,6 +301,22 @@ def bench_adjoint(args):
     del d, F
     api_host = {"value": round(n_env * args.steps / dt_host, 1), "ms_per_step": round(dt_host * 1e3 / args.steps, 3),
                 "note": "positions passed as a float64 host array: includes the host-to-device copy each step"}
+    # the same step with the atoms in random order (the edge pass groups centers along a
+    # Morton curve of their coordinates, so its speed should not depend on the order)
+    perm = np.random.default_rng(17).permutation(n_env)
+    sys_shuf = dict(system, positions=torch.tensor(np.asarray(system["positions"])[perm], device=dev),
+                    atomic_numbers=np.asarray(system["atomic_numbers"])[perm])
+    for _ in range(max(args.warmup, 1)):
+        d, F = sp.descriptor_and_vjp(sys_shuf, w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d, F = sp.descriptor_and_vjp(sys_shuf, w)
+    torch.cuda.synchronize()
+    dt_shuf = time.perf_counter() - t0
+    del d, F, sys_shuf
+    shuffled = {"value": round(n_env * args.steps / dt_shuf, 1), "ms_per_step": round(dt_shuf * 1e3 / args.steps, 3),
+                "note": "the same box with its atoms in random order"}
     # the same step with fixed-order force sums (opt-in: forces the same bits on every run)
     sp.ordered_forces = True
     log("timing", ":".join(chosen), "with ordered force sums")
@@ -348,6 +364,7 @@ def bench_adjoint(args):
                      "flops_per_launch": fl, "avg_kernel_ms": round(kern[dom], 4)},
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "api_host_positions": api_host,
+        "shuffled_atoms": shuffled,
         "ordered_force_sums": ordered,
         "forms": {form_name(f): {"value": round(r["value"], 1), "ms_per_step": round(r["ms_per_step"], 3),
                                  **{k: r[k] for k in ("desc_max_col_rel_err", "force_max_rel_err") if k in r},

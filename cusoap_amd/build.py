@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcusoap_amd.so")
 SOURCES = ["capi.hip", "neighbor.hip", "coeffs.hip", "spectrum.hip", "jacobian.hip", "adjoint.hip", "surrogate.hip",
-           "chebyshev.hip"]
+           "chebyshev.hip", "order.hip"]
 # compiled once per l_max value (-DCUSOAP_ONLY_L=l, that degree's kernels) plus once for the rest
 PER_DEGREE = ("coeffs.hip", "adjoint.hip", "chebyshev.hip")
 MAX_L = 9

@@ -429,7 +429,8 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                                                       const int32_t* __restrict__ center_species,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr, const TM* __restrict__ M,
-                                                      int n_centers, int ngrp, double* __restrict__ grad, VjpOrder ord) {
+                                                      const int32_t* __restrict__ order, int n_centers, int ngrp,
+                                                      int per_xcd, double* __restrict__ grad, VjpOrder ord) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     constexpr int NT = 256;
     constexpr bool MIXED = RAD == kRadMixed;
@@ -446,7 +447,9 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     TM* rad0 = (TM*)(qd + (MIXED && !FIXN ? n * NT : 0));   // [n][NT]   (runtime n)
     TM* rad1 = rad0 + (FIXN ? 0 : n * NT);                  // [n][NT]   (runtime n)
     TM* Ms = rad1 + (FIXN ? 0 : n * NT);                    // [ngrp][S][L2][n]
-    __shared__ int64_t goff[kVjpMaxGroup + 1];              // the group's edge offsets
+    __shared__ int gid[kVjpMaxGroup];                       // the group's centers (order[c0 + i])
+    __shared__ int64_t gbeg[kVjpMaxGroup];                  // first edge of each
+    __shared__ int64_t goff[kVjpMaxGroup + 1];              // their edges' offsets in the group's flat range
     __shared__ double gcen[kVjpMaxGroup][3];
     __shared__ int gslot[kVjpMaxGroup];                        // table slot of each center
     __shared__ double red[NT / 64][kVjpMaxGroup][3];           // center-attach sums per wavefront
@@ -461,49 +464,72 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             hkey[i] = -1;
             hval[i][0] = hval[i][1] = hval[i][2] = 0.0f;
         }
-    const int c0 = blockIdx.x * ngrp, ng = n_centers - c0 < ngrp ? n_centers - c0 : ngrp;
+    // the group: centers c0 .. c0 + ng - 1 of the order (the identity without one);
+    // consecutive workgroups go to different XCDs, each XCD taking a contiguous range of
+    // groups, whose neighbourhoods then share its L2
+    const int grp = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    const int c0 = grp * ngrp;
+    if (c0 >= n_centers) return;
+    const int ng = n_centers - c0 < ngrp ? n_centers - c0 : ngrp;
     const int per = S * n * L2;
-    if (tid <= ng) goff[tid] = offsets[c0 + tid];
     if (tid < ng) {
+        const int c = order ? order[c0 + tid] : c0 + tid;
+        gid[tid] = c;
+        gbeg[tid] = offsets[c];
+        goff[tid + 1] = offsets[c + 1] - offsets[c];            // counts; the prefix follows
         for (int d = 0; d < 3; ++d) {
-            gcen[tid][d] = centers[3 * (size_t)(c0 + tid) + d];
+            gcen[tid][d] = centers[3 * (size_t)c + d];
             for (int w = 0; w < NT / 64; ++w) red[w][tid][d] = 0.0;
         }
-        gslot[tid] = (RAD == kRadTable && center_species[c0 + tid] >= 0) ? center_species[c0 + tid] : S;
-    }
-    {
-        // the group's coefficient rows: 16-byte loads, four in flight per thread
-        const TM* src = M + (size_t)c0 * per;
-        const int tot = ng * per;
-        constexpr int V = 16 / sizeof(TM);
-        const int nv = (((size_t)c0 * per) % V == 0) ? tot / V : 0;   // rows of whole centers stay 16-byte aligned
-        using Vec = typename std::conditional<sizeof(TM) == 4, float4, double2>::type;
-        const Vec* s4 = (const Vec*)src;
-        Vec* d4 = (Vec*)Ms;
-        int i = tid;
-        for (; i + 3 * NT < nv; i += 4 * NT) {
-            const Vec a0 = s4[i], a1 = s4[i + NT], a2 = s4[i + 2 * NT], a3 = s4[i + 3 * NT];
-            d4[i] = a0;
-            d4[i + NT] = a1;
-            d4[i + 2 * NT] = a2;
-            d4[i + 3 * NT] = a3;
-        }
-        for (; i < nv; i += NT) d4[i] = s4[i];
-        for (int j = nv * V + tid; j < tot; j += NT) Ms[j] = src[j];
+        gslot[tid] = (RAD == kRadTable && center_species[c] >= 0) ? center_species[c] : S;
     }
     __syncthreads();
-    const int64_t E0 = goff[0], E1 = goff[ng];
+    if (tid == 0) {
+        goff[0] = 0;
+        for (int i = 0; i < ng; ++i) goff[i + 1] += goff[i];
+    }
+    {
+        // the group's coefficient rows, center by center: 16-byte loads (rows whose
+        // length is a multiple of 16 bytes), four in flight per thread
+        constexpr int V = 16 / sizeof(TM);
+        using Vec = typename std::conditional<sizeof(TM) == 4, float4, double2>::type;
+        if (per % V == 0) {
+            const int pv = per / V, nv = ng * pv;
+            Vec* d4 = (Vec*)Ms;
+            auto src = [&](int i) {
+                const int k = i / pv;
+                return ((const Vec*)(M + (size_t)gid[k] * per))[i - k * pv];
+            };
+            int i = tid;
+            for (; i + 3 * NT < nv; i += 4 * NT) {
+                const Vec a0 = src(i), a1 = src(i + NT), a2 = src(i + 2 * NT), a3 = src(i + 3 * NT);
+                d4[i] = a0;
+                d4[i + NT] = a1;
+                d4[i + 2 * NT] = a2;
+                d4[i + 3 * NT] = a3;
+            }
+            for (; i < nv; i += NT) d4[i] = src(i);
+        } else {
+            for (int j = tid; j < ng * per; j += NT) {
+                const int k = j / per;
+                Ms[j] = M[(size_t)gid[k] * per + (j - k * per)];
+            }
+        }
+    }
+    __syncthreads();
+    const int64_t E1 = goff[ng];
     TM* r0c = rad0 + tid;
     TM* r1c = rad1 + tid;
     double* qc = qd + tid;
     // the group's edges as one flat range (lanes stay busy across center boundaries)
-    for (int64_t base = E0; base < E1; base += NT) {
-        const int64_t e = base + tid;
+    for (int64_t base = 0; base < E1; base += NT) {
+        const int64_t f = base + tid;
         int ci = -1;
         TM gx = TM(0), gy = TM(0), gz = TM(0);
-        if (e < E1) {
+        if (f < E1) {
             ci = 0;
-            while (ci + 1 < ng && goff[ci + 1] <= e) ++ci;
+            while (ci + 1 < ng && goff[ci + 1] <= f) ++ci;
+            const int64_t e = gbeg[ci] + (f - goff[ci]);
             const double cx = gcen[ci][0], cy = gcen[ci][1], cz = gcen[ci][2];
             const int cslot = gslot[ci];
             const int ext = nbr[e];
@@ -737,7 +763,7 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                 for (int d = 0; d < 3; ++d) atomicAdd(&grad[3 * (size_t)a + d], (double)hval[i][d]);
         }
     if (center_attach && tid < ng) {
-        const int c = c0 + tid, ca = center_attach[c];
+        const int c = gid[tid], ca = center_attach[c];
         if (ca >= 0) {
             double sum[3];
             for (int d = 0; d < 3; ++d) sum[d] = ((red[0][tid][d] + red[1][tid][d]) + red[2][tid][d]) + red[3][tid][d];
@@ -758,7 +784,8 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
                                                       const int32_t* __restrict__ center_attach,
                                                       const int64_t* __restrict__ offsets,
                                                       const int32_t* __restrict__ nbr, const double* __restrict__ M,
-                                                      int n_centers, double* __restrict__ grad, VjpOrder ord) {
+                                                      const int32_t* __restrict__ order, int n_centers,
+                                                      double* __restrict__ grad, VjpOrder ord) {
     constexpr int L1 = LMAX + 1, L2 = L1 * L1;
     const int n = b.n_max, S = b.n_species, NL = n * L1;
     const bool ordered = vjp_ordered(ord, offsets, n_centers);
@@ -767,7 +794,7 @@ __global__ __launch_bounds__(64) void vjp_poly_kernel(cusoap_basis_t b, const do
     double* Ms = (double*)smem;               // [S][L2][n]
     double* R0 = Ms + S * n * L2;             // [64][NL]
     double* R1 = R0 + 64 * NL;                // [64][NL]
-    const int c = blockIdx.x, lane = threadIdx.x;
+    const int c = order ? order[blockIdx.x] : (int)blockIdx.x, lane = threadIdx.x;
     const int per = S * n * L2;
     for (int i = lane; i < per; i += 64) Ms[i] = M[(size_t)c * per + i];
     __syncthreads();
@@ -880,7 +907,7 @@ template <int LMAX, typename TM, int RAD, int NN>
 int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
                    const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
                    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
-                   double* grad, const VjpOrder& ord, hipStream_t st) {
+                   const int32_t* order, double* grad, const VjpOrder& ord, hipStream_t st) {
     const size_t cols = sizeof(TM) * (NN > 0 ? 0 : 2 * 256 * (size_t)b.n_max) +
                         (RAD == kRadMixed && NN == 0 ? sizeof(double) * 256 * (size_t)b.n_max : 0);
     const size_t per_bytes = sizeof(TM) * (size_t)b.n_species * b.n_max * (LMAX + 1) * (LMAX + 1);
@@ -895,9 +922,9 @@ int launch_vjp_gto(const cusoap_basis_t& b, const double* pos, int n_atoms, cons
     if (smem > 64 * 1024)
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_edges", st);
-    kern<<<(n_centers + ngrp - 1) / ngrp, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach,
-                                                           center_species, offsets, nbr, (const TM*)M, n_centers,
-                                                           ngrp, grad, ord);
+    const int groups = (n_centers + ngrp - 1) / ngrp, per_xcd = (groups + 7) / 8;
+    kern<<<8 * per_xcd, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, center_species,
+                                         offsets, nbr, (const TM*)M, order, n_centers, ngrp, per_xcd, grad, ord);
     prof_end("vjp_edges", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -907,8 +934,8 @@ template <int LMAX>
 int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
                      const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
                      const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,
-                     double* grad, const VjpOrder& ord, hipStream_t st) {
-#define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, grad, ord, st
+                     const int32_t* order, double* grad, const VjpOrder& ord, hipStream_t st) {
+#define CUSOAP_VJP_ARGS b, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, M, order, grad, ord, st
     if (b.rad_table) {
         if (prec == CUSOAP_PREC_F64) return launch_vjp_gto<LMAX, double, kRadTable, 0>(CUSOAP_VJP_ARGS);
         // registers for the radial vectors (float table form) at n_max 4..8
@@ -936,7 +963,7 @@ int launch_vjp_edges(const cusoap_basis_t& b, int prec, const double* pos, int n
         CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     prof_begin("vjp_edges", st);
     kern<<<n_centers, 64, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, center_attach, offsets, nbr,
-                                      (const double*)M, n_centers, grad, ord);
+                                      (const double*)M, order, n_centers, grad, ord);
     prof_end("vjp_edges", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;
@@ -1039,8 +1066,8 @@ int launch_adjoint(const cusoap_basis_t& b, int mode, const void* coeffs, const 
 // points, which call the per-degree functions).
 #define CUSOAP_VJP_PARAMS const cusoap_basis_t& b, int prec, const double* pos, int n_atoms, const int32_t* atom_sp, \
     const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,                     \
-    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M, double* grad,          \
-    const VjpOrder& ord, hipStream_t st
+    const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* M,                       \
+    const int32_t* order, double* grad, const VjpOrder& ord, hipStream_t st
 #define CUSOAP_VJP_DECL(L) int vjp_edges_l##L(CUSOAP_VJP_PARAMS);
 CUSOAP_VJP_DECL(0) CUSOAP_VJP_DECL(1) CUSOAP_VJP_DECL(2) CUSOAP_VJP_DECL(3) CUSOAP_VJP_DECL(4)
 CUSOAP_VJP_DECL(5) CUSOAP_VJP_DECL(6) CUSOAP_VJP_DECL(7) CUSOAP_VJP_DECL(8) CUSOAP_VJP_DECL(9)
@@ -1050,12 +1077,23 @@ CUSOAP_VJP_DECL(5) CUSOAP_VJP_DECL(6) CUSOAP_VJP_DECL(7) CUSOAP_VJP_DECL(8) CUSO
 #define CUSOAP_VJP_DEF(L)                                                                                        \
     int vjp_edges_l##L(CUSOAP_VJP_PARAMS) {                                                                      \
         return launch_vjp_edges<L>(b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach,     \
-                                   center_species, offsets, nbr, M, grad, ord, st);                              \
+                                   center_species, offsets, nbr, M, order, grad, ord, st);                       \
     }
 #define CUSOAP_VJP_DEF2(L) CUSOAP_VJP_DEF(L)
 CUSOAP_VJP_DEF2(CUSOAP_ONLY_L)
 #else
-size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) { return vjp_m_bytes(b, n_centers) + vjp_w_bytes(b); }
+size_t center_order_ws_bytes(int n);                        // order.hip
+int center_order(const double* centers, int n, void* ws, int32_t* order, hipStream_t st);
+
+// dL/dc rows (M), the shared weight matrix (W), then the centers' Morton order and the
+// workspace that sorts it
+size_t vjp_core_ws_bytes(const cusoap_basis_t* b, int n_centers) { return vjp_m_bytes(b, n_centers) + vjp_w_bytes(b); }
+size_t vjp_order_bytes(int n_centers) {
+    return align_up(sizeof(int32_t) * (size_t)std::max(n_centers, 1), 256) + center_order_ws_bytes(n_centers);
+}
+size_t vjp_ws_bytes(const cusoap_basis_t* b, int n_centers) {
+    return vjp_core_ws_bytes(b, n_centers) + vjp_order_bytes(n_centers);
+}
 
 namespace {
 
@@ -1193,7 +1231,8 @@ int vjp_core(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
              const double* shifts, const double* centers, int n_centers, const int32_t* center_attach,
              const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs,
              int coeffs_dtype, const int32_t* pair_feature, const void* gd, int gd_dtype, int gd_broadcast,
-             int n_feat, void* ws, double* grad, const VjpOrder& ord, hipStream_t st) {
+             int n_feat, void* ws, const int32_t* order, hipEvent_t order_ready, double* grad, const VjpOrder& ord,
+             hipStream_t st) {
     const int fold = (b->rbf == CUSOAP_RBF_GTO && prec != CUSOAP_PREC_MIXED && !b->rad_table) ? 1 : 0;
     const int fold_k = (b->rbf == CUSOAP_RBF_GTO || b->rad_table) ? 1 : 0;   // the GTO / table edge kernels
     int rc;
@@ -1209,8 +1248,9 @@ int vjp_core(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
     }
 #undef CUSOAP_ADJ
     if (rc) return rc;
+    if (order_ready) CUSOAP_HIP(hipStreamWaitEvent(st, order_ready, 0));
 #define CUSOAP_CASE(L) \
-    case L: return vjp_edges_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, grad, ord, st);
+    case L: return vjp_edges_l##L(*b, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species, offsets, nbr, ws, order, grad, ord, st);
     switch (b->l_max) {
         CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
         CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
@@ -1249,6 +1289,50 @@ int scratch_address(void** out) {
 
 }  // namespace
 
+// A second stream per device (default priority, non-blocking) for the centers' Morton
+// order: it depends on the coordinates only, so its small kernels run beside the
+// adjoint's first kernel; the edge pass waits on an event.
+std::mutex g_side_mu;
+std::map<int, hipStream_t> g_side;
+
+int side_stream(hipStream_t* out) {
+    int dev = 0;
+    CUSOAP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    auto it = g_side.find(dev);
+    if (it == g_side.end()) {
+        hipStream_t s = nullptr;
+        CUSOAP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        it = g_side.emplace(dev, s).first;
+    }
+    *out = it->second;
+    return CUSOAP_OK;
+}
+
+// the centers' Morton order, in the workspace after the dL/dc rows (see vjp_ws_bytes), on
+// the side stream after the work already on st; *ready is recorded when it is done (the
+// caller waits on it, then destroys it)
+int morton_order(const cusoap_basis_t* b, const double* centers, int n_centers, void* ws, int32_t** order,
+                 hipEvent_t* ready, hipStream_t st) {
+    char* p = (char*)ws + vjp_core_ws_bytes(b, n_centers);
+    *order = (int32_t*)p;
+    *ready = nullptr;
+    hipStream_t side = nullptr;
+    int rc = side_stream(&side);
+    if (rc) return rc;
+    hipEvent_t fork = nullptr;
+    CUSOAP_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    CUSOAP_HIP(hipEventRecord(fork, st));
+    const hipError_t we = hipStreamWaitEvent(side, fork, 0);
+    (void)hipEventDestroy(fork);             // released once recorded work completes
+    if (we != hipSuccess) return fail(CUSOAP_ERR_HIP, std::string("vjp: stream wait: ") + hipGetErrorString(we));
+    if ((rc = center_order(centers, n_centers, p + align_up(sizeof(int32_t) * (size_t)n_centers, 256), *order, side)))
+        return rc;
+    CUSOAP_HIP(hipEventCreateWithFlags(ready, hipEventDisableTiming));
+    CUSOAP_HIP(hipEventRecord(*ready, side));
+    return CUSOAP_OK;
+}
+
 int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
              const double* shifts, int n_img, const double* centers, int n_centers, const int32_t* center_attach,
              const int32_t* center_species, const int64_t* offsets, const int32_t* nbr, const void* coeffs, int coeffs_dtype,
@@ -1260,9 +1344,14 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
     if (ws_bytes < vjp_ws_bytes(b, n_centers)) return fail(CUSOAP_ERR_WORKSPACE, "vjp: workspace too small");
     if ((rc = zero_f64(grad, 3 * (int64_t)n_atoms, st))) return rc;
     if (n_centers == 0) return CUSOAP_OK;
-    return vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
-                    offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad,
-                    VjpOrder{nullptr, nullptr, nullptr, 0}, st);
+    int32_t* order = nullptr;
+    hipEvent_t ready = nullptr;
+    if ((rc = morton_order(b, centers, n_centers, ws, &order, &ready, st))) return rc;
+    rc = vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
+                  offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, order, ready,
+                  grad, VjpOrder{nullptr, nullptr, nullptr, 0}, st);
+    if (ready) (void)hipEventDestroy(ready);
+    return rc;
 }
 
 size_t vjp_ordered_ws_bytes(const cusoap_basis_t* b, int prec, int n_centers, int64_t n_edges) {
@@ -1305,10 +1394,15 @@ int vjp_ordered_impl(const cusoap_basis_t* b, int mode, int prec, const double* 
         return fail(CUSOAP_ERR_WORKSPACE, "vjp_ordered: workspace too small");
     if ((rc = zero_f64(grad, 3 * (int64_t)n_atoms, st))) return rc;
     if (n_centers == 0) return CUSOAP_OK;
+    int32_t* order = nullptr;
+    hipEvent_t ready = nullptr;
+    if ((rc = morton_order(b, centers, n_centers, ws, &order, &ready, st))) return rc;
     char* p = (char*)ws + base;
     const VjpOrder ord{rev, p + order_center_bytes(n_centers), (double*)p, n_edges};
     rc = vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
-                  offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, grad, ord, st);
+                  offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, order, ready,
+                  grad, ord, st);
+    if (ready) (void)hipEventDestroy(ready);
     if (rc) return rc;
     prof_begin("vjp_gather", st);
     if (prec == CUSOAP_PREC_F64)
@@ -1337,8 +1431,8 @@ int vjp_noalloc_impl(const cusoap_basis_t* b, int mode, int prec, const double* 
     const int L2 = (b->l_max + 1) * (b->l_max + 1);
     const size_t row_f64 = sizeof(double) * (size_t)b->n_species * b->n_max * L2;
     int chunk = (int)std::min<size_t>((kNoAllocScratchBytes - wbytes - 256) / row_f64, (size_t)INT32_MAX);
-    while (chunk > 1 && vjp_ws_bytes(b, chunk) > kNoAllocScratchBytes) --chunk;
-    if (chunk < 1 || vjp_ws_bytes(b, chunk) > kNoAllocScratchBytes)
+    while (chunk > 1 && vjp_core_ws_bytes(b, chunk) > kNoAllocScratchBytes) --chunk;
+    if (chunk < 1 || vjp_core_ws_bytes(b, chunk) > kNoAllocScratchBytes)
         return fail(CUSOAP_ERR_CAPACITY, "vjp_noalloc: basis too large for the 64 MiB scratch");
     void* ws = nullptr;
     if ((rc = scratch_address(&ws))) return rc;
@@ -1351,7 +1445,7 @@ int vjp_noalloc_impl(const cusoap_basis_t* b, int mode, int prec, const double* 
                       center_attach ? center_attach + c0 : nullptr, center_species ? center_species + c0 : nullptr,
                       offsets + c0, nbr, (const char*)coeffs + csz * per * c0, coeffs_dtype, pair_feature,
                       gd_broadcast ? gd : (const void*)((const char*)gd + gsz * (size_t)n_feat * c0), gd_dtype,
-                      gd_broadcast, n_feat, ws, grad, VjpOrder{nullptr, nullptr, nullptr, 0}, st);
+                      gd_broadcast, n_feat, ws, nullptr, nullptr, grad, VjpOrder{nullptr, nullptr, nullptr, 0}, st);
         if (rc) return rc;
     }
     return CUSOAP_OK;

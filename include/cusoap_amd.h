@@ -231,7 +231,11 @@ int cusoap_jacobian_dense(const cusoap_basis_t* basis, int32_t mode,
  * -1, required when basis->rad_table is set (table slot), else may be NULL.
  * Forces are accumulated with float64 atomics (the summation order, and so the
  * last bits, vary from run to run); cusoap_vjp_ordered below sums in a fixed
- * order. */
+ * order.  The workspace also holds the centers' Morton order (a sort of their
+ * coordinates, run on a second stream of the library's own beside the first
+ * kernel): the edge pass takes groups of centers along it, so its speed does
+ * not depend on the order of the caller's atoms (cusoap_vjp_noalloc keeps the
+ * given order). */
 int cusoap_vjp_workspace_size(const cusoap_basis_t* basis, int32_t n_centers, size_t* bytes);
 int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
                const double* positions, int32_t n_atoms, const int32_t* atom_species,

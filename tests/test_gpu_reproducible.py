@@ -139,3 +139,23 @@ def test_edge_reverse_is_the_reverse_edge(system):
     say(f"edge_reverse {system}: {len(nbr)} edges, {len(sh)} images, {int((~matched).sum())} without reverse")
     assert np.array_equal(r, want)
     assert np.array_equal(r[r[matched]], np.nonzero(matched)[0])
+
+
+@pytest.mark.parametrize("prec,radial,tol", [("float64", "exact", 1e-12), ("mixed", "table", 1e-6)])
+def test_forces_do_not_depend_on_atom_order(prec, radial, tol):
+    """The edge pass groups centers along a Morton curve of their coordinates: the same box
+    with its atoms shuffled gives the same forces (permuted back) to rounding of the sums."""
+    s = systems.water_box(6, seed=9)                     # 648 atoms
+    p = dict(P, r_cut=5.0)
+    g = SOAP(device=DEV, dtype="float64", precision=prec, radial=radial, **p)
+    n = len(s["positions"])
+    w = torch.tensor(np.random.default_rng(2).normal(size=g.get_number_of_features()), device=DEV)
+    F = g.vjp(s, w).cpu().numpy()
+    perm = np.random.default_rng(8).permutation(n)
+    sp = dict(s, positions=np.asarray(s["positions"])[perm], atomic_numbers=np.asarray(s["atomic_numbers"])[perm])
+    Fp = g.vjp(sp, w).cpu().numpy()
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    e = ferr(Fp[inv], F)
+    say(f"shuffled atoms {prec}/{radial}: {e:.1e}")
+    assert e < tol
