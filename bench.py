@@ -458,7 +458,7 @@ def bench_dense(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profile("jacobian_dense_h2o1000_traffic.txt"),
-                     "kernel": "dense_kernel (jacobian_dense)", "bytes_per_launch": jac_bytes,
+                     "kernel": "dense_fused_kernel (jacobian_dense)", "bytes_per_launch": jac_bytes,
                      "avg_kernel_ms": round(kern["jacobian_dense"], 4)},
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
     }

@@ -16,7 +16,7 @@
 // with per-edge contractions U[kq][l] = sum_m Y_lm c[kq][lm] and
 // V_d[kq][l] = sum_m dY_lm/dr_d c[kq][lm].
 //
-// Three kernels:
+// Kernels:
 //  rowkey_kernel   one workgroup per center: the center's included edges as
 //                  (output row, edge) keys, bitonic-sorted in LDS -> global.
 //  records_kernel  workgroup (center, y) takes the center's edge tiles y, y + gy, ...
@@ -27,6 +27,9 @@
 //                  float32 when the rows are computed in float32, else float64:
 //                      rvec[3] | Bw[n][l] | Bt[n][l] | ps: U[Sn][l], V[3][Sn][l]
 //                                                      cc: Y[L2], dY[3][L2]
+//  dense_fused_kernel  (one image, no attached center, GTO, power spectrum: one edge
+//                  per output row) forms each row's record in LDS in the wavefront that
+//                  stores the row, instead of records_kernel + dense_kernel.
 //  dense_kernel    one workgroup per (center, 32-row chunk): each run of
 //                  consecutive rows without an edge is stored by the whole
 //                  workgroup as one flat range of 16-byte zeros; then each
@@ -595,6 +598,193 @@ __global__ __launch_bounds__(256) void dense_kernel(cusoap_basis_t b, int mode, 
     }
 }
 
+// Power-spectrum rows of the GTO basis when every output row has exactly one edge (one
+// image, no attached center: the reference's closed-form case): the wavefront that
+// stores a row first forms that edge's record in LDS, as records_kernel does (float64
+// arithmetic in the same order, stored as TC), so no record array goes through HBM.
+// Workgroup (center, 32-row chunk), zero runs and row stores as dense_kernel (VEC form).
+template <int LMAX, typename TO, typename TC>
+__global__ __launch_bounds__(256) void dense_fused_kernel(cusoap_basis_t b, const double* __restrict__ pos,
+                                                          int n_atoms, const int32_t* __restrict__ atom_sp,
+                                                          const double* __restrict__ shifts,
+                                                          const double* __restrict__ centers,
+                                                          const int64_t* __restrict__ offsets,
+                                                          const int32_t* __restrict__ nbr,
+                                                          const double* __restrict__ coeffs,
+                                                          const unsigned long long* __restrict__ rowkeys,
+                                                          const int32_t* __restrict__ chunk_lo, int n_inc, int chunks,
+                                                          const uint32_t* __restrict__ ft, int n_feat,
+                                                          TO* __restrict__ out) {
+    constexpr int L1 = LMAX + 1, L2 = L1 * L1;
+    __shared__ unsigned long long ckeys[kChunkKeys];
+    __shared__ int rstart[kRowsPerBlock + 1];
+    __shared__ int nzrows[kRowsPerBlock];
+    __shared__ int nnz;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = b.n_max, S = b.n_species, NL = n * L1;
+    const RecLayout R = rec_layout(S, n, LMAX, CUSOAP_MODE_PS);
+    double* prefs = (double*)smem;                                   // [16]
+    TC* prefc = (TC*)(prefs + 16);                                   // [16]
+    double* Cs = prefs + 32;                                         // [S n L2] the center's coefficients
+    double* Yw = Cs + S * n * L2;                                    // [4][4 L2] harmonics, gradients
+    double* Qw = Yw + 16 * L2;                                       // [4][NL]   GTO primitives
+    unsigned long long* plan = (unsigned long long*)(Qw + 4 * NL);   // [n_feat]
+    TC* recs = (TC*)(plan + n_feat);                                 // [4][stride]
+    const int c = blockIdx.x / chunks, ch = blockIdx.x - c * chunks;
+    const int r0 = ch * kRowsPerBlock;
+    const int r1 = (r0 + kRowsPerBlock < n_inc) ? r0 + kRowsPerBlock : n_inc;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t e0 = offsets[c];
+    const int lo = chunk_lo[(size_t)c * (chunks + 1) + ch], hi = chunk_lo[(size_t)c * (chunks + 1) + ch + 1];
+    const bool in_lds = hi - lo <= kChunkKeys;
+    const unsigned long long* keys = in_lds ? ckeys : rowkeys + e0 + lo;
+    if (in_lds)
+        for (int i = threadIdx.x; i < hi - lo; i += blockDim.x) ckeys[i] = rowkeys[e0 + lo + i];
+    for (int f = threadIdx.x; f < n_feat; f += blockDim.x) {
+        const uint32_t v = ft[f];
+        const int j = (int)(v & 0xffu), jd = (int)((v >> 8) & 0xffu);
+        const int nn = (int)((v >> 16) & 0x3fu), nd = (int)((v >> 22) & 0x3fu), l = (int)(v >> 28);
+        plan[f] = pack_plan(j, jd, l, nn * R.L1 + l, nd * R.L1 + l, (jd * n + nd) * R.L1 + l, (j * n + nn) * R.L1 + l);
+    }
+    if ((int)threadIdx.x <= LMAX) {
+        const double pf = sqrt(8.0 * M_PI * M_PI / (2 * threadIdx.x + 1));
+        prefs[threadIdx.x] = pf;
+        prefc[threadIdx.x] = (TC)pf;
+    }
+    for (int i = threadIdx.x; i < S * n * L2; i += blockDim.x) Cs[i] = coeffs[(size_t)c * S * n * L2 + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i <= r1 - r0; i += blockDim.x) rstart[i] = lower_row(keys, 0, hi - lo, r0 + i);
+    __syncthreads();
+    const int row_len = 3 * n_feat;
+    // 1) every run of consecutive rows without an edge: one flat block-wide store
+    if (threadIdx.x == 0) nnz = 0;
+    {
+        int r = r0;
+        while (r < r1) {
+            while (r < r1 && rstart[r - r0] != rstart[r - r0 + 1]) ++r;
+            const int rs = r;
+            while (r < r1 && rstart[r - r0] == rstart[r - r0 + 1]) ++r;
+            if (r > rs) {
+                typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+                const u32x4 z = {0u, 0u, 0u, 0u};
+                u32x4* p = (u32x4*)(out + ((size_t)c * n_inc + rs) * (size_t)row_len);
+                const size_t nv = (size_t)(r - rs) * row_len * sizeof(TO) / 16;
+                for (size_t i = threadIdx.x; i < nv; i += blockDim.x) p[i] = z;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)(r1 - r0)) {
+        const int r = r0 + threadIdx.x;
+        if (rstart[r - r0] != rstart[r - r0 + 1]) nzrows[atomicAdd(&nnz, 1)] = r;
+    }
+    __syncthreads();
+    // 2) rows with an edge: one wavefront per row, its record formed in the wavefront's slot
+    const double cx = centers[3 * (size_t)c], cy = centers[3 * (size_t)c + 1], cz = centers[3 * (size_t)c + 2];
+    const double inv_na = 1.0 / (double)n_atoms;
+    TC* myrec = recs + wv * R.stride;
+    double* Y = Yw + wv * 4 * L2;
+    double* Q = Qw + wv * NL;
+    for (int i = wv; i < nnz; i += 4) {
+        const int r = nzrows[i];
+        TO* dst = out + ((size_t)c * n_inc + r) * (size_t)row_len;
+        const int64_t e = e0 + (int64_t)(keys[rstart[r - r0]] & 0xffffffffull);
+        int img, a;
+        split_ext(nbr[e], n_atoms, inv_na, img, a);
+        const double rx = (pos[3 * (size_t)a] + shifts[3 * img]) - cx;
+        const double ry = (pos[3 * (size_t)a + 1] + shifts[3 * img + 1]) - cy;
+        const double rz = (pos[3 * (size_t)a + 2] + shifts[3 * img + 2]) - cz;
+        const double r2 = (rx * rx + ry * ry) + rz * rz;
+        const double rr = sqrt(r2);
+        double w = 1.0, dw = 0.0;
+        if (b.weighting != CUSOAP_W_NONE) weighting(b, rr, w, dw);
+        const double dwr = rr > 0.0 ? dw / rr : 0.0;
+        if (lane == 0)
+            solid_harmonics<LMAX, true, double>(rx, ry, rz, [&](int lm, double v, double gx, double gy, double gz) {
+                Y[lm] = v;
+                Y[L2 + lm] = gx;
+                Y[2 * L2 + lm] = gy;
+                Y[3 * L2 + lm] = gz;
+            });
+        for (int t = lane; t < NL; t += 64) {
+            const int k = t / L1, l = t - k * L1;
+            Q[t] = b.gto_pref[l * n + k] * exp(-b.gto_gamma[l * n + k] * r2);
+        }
+        if (lane < 3) myrec[lane] = (TC)(lane == 0 ? rx : (lane == 1 ? ry : rz));
+        wave_sync();
+        for (int t = lane; t < NL; t += 64) {
+            const int nn = t / L1, l = t - nn * L1;
+            const double* beta = b.gto_beta + ((size_t)l * n + nn) * n;
+            double B = 0.0, Bh = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const double q = Q[k * L1 + l];
+                B += beta[k] * q;
+                Bh += beta[k] * (-2.0 * b.gto_gamma[l * n + k]) * q;
+            }
+            myrec[R.offB + t] = (TC)(w * B);
+            myrec[R.offBt + t] = (TC)(w * Bh + dwr * B);
+        }
+        for (int t = lane; t < 4 * R.SNL; t += 64) {
+            const int comp = t / R.SNL, rem = t - comp * R.SNL;
+            const int kq = rem / L1, l = rem - kq * L1;
+            const double* yv = Y + comp * L2 + l * l;
+            const double* cr = Cs + kq * L2 + l * l;
+            double acc = 0.0;
+            for (int m = 0; m < 2 * l + 1; ++m) acc += yv[m] * cr[m];
+            myrec[R.offU + t] = (TC)acc;
+        }
+        wave_sync();
+        const int s = atom_sp[a];
+        int t = 4 * lane;
+        int d = t / n_feat, f = t - d * n_feat;
+        for (; t < row_len; t += 256) {
+            TC v[4];
+            int dd = d, ff = f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (ff >= n_feat) {
+                    ff -= n_feat;
+                    ++dd;
+                }
+                v[k] = ps_elem<TC>(myrec, R, s, dd, plan[ff], prefc);
+                ++ff;
+            }
+            store4<TO>(dst + t, (double)v[0], (double)v[1], (double)v[2], (double)v[3]);
+            f += 256;
+            while (f >= n_feat) {
+                f -= n_feat;
+                ++d;
+            }
+        }
+        wave_sync();
+    }
+}
+
+template <int LMAX, typename TO, typename TC>
+int launch_dense_fused(const cusoap_basis_t& b, const double* pos, int n_atoms, const int32_t* atom_sp,
+                       const double* shifts, const double* centers, const int64_t* offsets, const int32_t* nbr,
+                       const double* coeffs, const unsigned long long* rowkeys, const int32_t* chunk_lo, int n_centers,
+                       int n_inc, const uint32_t* ft, int n_feat, void* out, hipStream_t st) {
+    constexpr int L1 = LMAX + 1, L2 = L1 * L1;
+    const RecLayout R = rec_layout(b.n_species, b.n_max, LMAX, CUSOAP_MODE_PS);
+    if (R.SNL >= 4096 || R.NL >= 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: too many radial channels");
+    const size_t smem = sizeof(double) * (32 + (size_t)b.n_species * b.n_max * L2 + 16 * L2 + 4 * (size_t)R.NL) +
+                        sizeof(unsigned long long) * (size_t)n_feat + sizeof(TC) * 4 * (size_t)R.stride;
+    if (smem > 160 * 1024) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: record too large for LDS");
+    const int chunks = (n_inc + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int64_t grid = (int64_t)n_centers * chunks;
+    if (grid >= ((int64_t)1 << 31)) return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: output too large");
+    auto kern = dense_fused_kernel<LMAX, TO, TC>;
+    if (smem > 64 * 1024)
+        CUSOAP_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    prof_begin("jacobian_dense", st);
+    kern<<<(unsigned)grid, 256, smem, st>>>(b, pos, n_atoms, atom_sp, shifts, centers, offsets, nbr, coeffs, rowkeys,
+                                            chunk_lo, n_inc, chunks, ft, n_feat, (TO*)out);
+    prof_end("jacobian_dense", st);
+    CUSOAP_LAUNCH_CHECK();
+    return CUSOAP_OK;
+}
+
 int records_tile(const RecLayout& R, int S, int n, size_t* smem) {
     const size_t fixed = (size_t)S * n * R.L2;
     const size_t per = 4 * (size_t)R.L2 + 2 * (size_t)R.NL + 8;
@@ -701,7 +891,6 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
                         const int32_t* center_attach, const int64_t* offsets, const int32_t* nbr, int64_t n_edges,
                         const double* coeffs, const int32_t* inv_atom, int n_inc, const uint32_t* ft, int n_feat,
                         void* ws, size_t ws_bytes, void* out, int odt, int cdt, hipStream_t st) {
-    (void)n_img;
     if (!b || b->n_max < 1 || b->n_max > kMaxN || b->l_max < 0 || b->l_max > kMaxL)
         return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: n_max must be 1..16 and l_max 0..9");
     if (mode != CUSOAP_MODE_PS && mode != CUSOAP_MODE_CC) return fail(CUSOAP_ERR_ARG, "jacobian: unknown mode");
@@ -718,6 +907,32 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
     int rc = CUSOAP_OK;
     // records in float32 when the rows are computed in float32 (half the record traffic)
     const bool f32 = odt == CUSOAP_DT_F32 && cdt == CUSOAP_DT_F32;
+    // 4-element groups per lane: every row length must be a multiple of 4 elements
+    const bool vec = ((3 * (size_t)n_feat) % 4 == 0) && (((uintptr_t)out & 15) == 0);
+    if (n_img == 1 && !center_attach && b->rbf == CUSOAP_RBF_GTO && mode == CUSOAP_MODE_PS && vec) {
+        // one edge per output row: each row's record is formed where the row is stored
+#define CUSOAP_FUSED(L, TO, TC)                                                                                   \
+    launch_dense_fused<L, TO, TC>(*b, pos, n_atoms, atom_sp, shifts, centers, offsets, nbr, coeffs, w.rowkeys,      \
+                                  w.chunk_lo, n_centers, n_inc, ft, n_feat, out, st)
+#define CUSOAP_CASE(L)                                                                                           \
+    case L:                                                                                                      \
+        rc = f32 ? CUSOAP_FUSED(L, float, float)                                                                 \
+                 : (odt == CUSOAP_DT_F32 ? CUSOAP_FUSED(L, float, double) : CUSOAP_FUSED(L, double, double));    \
+        break;
+        switch (b->l_max) {
+            CUSOAP_CASE(0) CUSOAP_CASE(1) CUSOAP_CASE(2) CUSOAP_CASE(3) CUSOAP_CASE(4)
+            CUSOAP_CASE(5) CUSOAP_CASE(6) CUSOAP_CASE(7) CUSOAP_CASE(8) CUSOAP_CASE(9)
+            default: return fail(CUSOAP_ERR_UNSUPPORTED, "jacobian: l_max out of range");
+        }
+#undef CUSOAP_CASE
+#undef CUSOAP_FUSED
+        if (rc) return rc;
+        int32_t herr = 0;
+        CUSOAP_HIP(hipMemcpyAsync(&herr, w.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        CUSOAP_HIP(hipStreamSynchronize(st));
+        if (herr) return fail(CUSOAP_ERR_CAPACITY, "jacobian: a center has more than 4096 included edges");
+        return CUSOAP_OK;
+    }
 #define CUSOAP_CASE(L) \
     case L: \
         rc = f32 ? launch_records<L, float>(*b, mode, pos, n_atoms, shifts, centers, n_centers, offsets, nbr, n_edges, \
@@ -732,8 +947,6 @@ int jacobian_dense_impl(const cusoap_basis_t* b, int mode, const double* pos, in
     }
 #undef CUSOAP_CASE
     if (rc) return rc;
-    // 4-element groups per lane: every row length must be a multiple of 4 elements
-    const bool vec = ((3 * (size_t)n_feat) % 4 == 0) && (((uintptr_t)out & 15) == 0);
 #define CUSOAP_DENSE(TO, TC, V) \
     launch_dense<TO, TC, V>(*b, mode, n_atoms, atom_sp, center_attach, offsets, nbr, inv_atom, w.rowkeys, w.chunk_lo, \
                             n_centers, n_inc, ft, n_feat, (const TC*)w.rec, out, st)
