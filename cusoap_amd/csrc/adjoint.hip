@@ -518,6 +518,9 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
     }
     __syncthreads();
     const int64_t E1 = goff[ng];
+    // table interval width and its inverse (once; the edges multiply)
+    const double hd = RAD == kRadTable ? b.rad_rmax / b.rad_intervals : 0.0;
+    const double ihd = RAD == kRadTable ? b.rad_intervals / b.rad_rmax : 0.0;
     TM* r0c = rad0 + tid;
     TM* r1c = rad1 + tid;
     double* qc = qd + tid;
@@ -542,7 +545,8 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             const double r = sqrt(r2);
             double w = 1.0, dw = 0.0;
             if (b.weighting != CUSOAP_W_NONE) weighting(b, r, w, dw);
-            const double dwr = r > 0.0 ? dw / r : 0.0;
+            const double ir = r > 0.0 ? 1.0 / r : 0.0;            // the edge's one division
+            const double dwr = dw * ir;
             const TM* Mrow = Ms + (size_t)ci * per + (size_t)atom_sp[a] * L2 * n;
             const TM trx = (TM)rx, tryy = (TM)ry, trz = (TM)rz;
             // table interpolation weights: value (v*) and (1/r) d/dr (c*)
@@ -550,7 +554,7 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
             TM v00 = TM(0), v10 = TM(0), v01 = TM(0), v11 = TM(0), c00 = TM(0), c10 = TM(0), c01 = TM(0), c11 = TM(0);
             if (RAD == kRadTable) {
                 const int G = b.rad_intervals;
-                const double hd = b.rad_rmax / G, xr = r / hd;
+                const double xr = r * ihd;
                 int i = (int)xr;
                 i = i < G - 1 ? i : G - 1;
                 const double t = xr - i, t2 = t * t, t3 = t2 * t;
@@ -560,14 +564,13 @@ __global__ __launch_bounds__(256, vjp_min_waves<NN>()) void vjp_gto_kernel(cusoa
                 v11 = (TM)(hd * (t3 - t2));
                 if (i == 0) {
                     // the node slope at r = 0 is zero, so P'(r)/r is a polynomial on the first interval
-                    c00 = (TM)(6.0 * (t - 1.0) / (hd * hd));
+                    c00 = (TM)(6.0 * (t - 1.0) * (ihd * ihd));
                     c01 = -c00;
-                    c11 = (TM)((3.0 * t - 2.0) / hd);
+                    c11 = (TM)((3.0 * t - 2.0) * ihd);
                 } else {
-                    const double ir = 1.0 / r;
-                    c00 = (TM)((6.0 * t2 - 6.0 * t) / hd * ir);
+                    c00 = (TM)((6.0 * t2 - 6.0 * t) * ihd * ir);
                     c10 = (TM)((3.0 * t2 - 4.0 * t + 1.0) * ir);
-                    c01 = (TM)((6.0 * t - 6.0 * t2) / hd * ir);
+                    c01 = -c00;
                     c11 = (TM)((3.0 * t2 - 2.0 * t) * ir);
                 }
                 tb = b.rad_table + ((size_t)(cslot * S + atom_sp[a]) * (G + 1) + i) * (L1 * 2 * n);
