@@ -301,35 +301,39 @@ def bench_adjoint(args):
     del d, F
     api_host = {"value": round(n_env * args.steps / dt_host, 1), "ms_per_step": round(dt_host * 1e3 / args.steps, 3),
                 "note": "positions passed as a float64 host array: includes the host-to-device copy each step"}
-    # the same step with the atoms in random order (the edge pass groups centers along a
-    # Morton curve of their coordinates, so its speed should not depend on the order)
-    perm = np.random.default_rng(17).permutation(n_env)
-    sys_shuf = dict(system, positions=torch.tensor(np.asarray(system["positions"])[perm], device=dev),
-                    atomic_numbers=np.asarray(system["atomic_numbers"])[perm])
-    for _ in range(max(args.warmup, 1)):
-        d, F = sp.descriptor_and_vjp(sys_shuf, w)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        d, F = sp.descriptor_and_vjp(sys_shuf, w)
-    torch.cuda.synchronize()
-    dt_shuf = time.perf_counter() - t0
-    del d, F, sys_shuf
-    shuffled = {"value": round(n_env * args.steps / dt_shuf, 1), "ms_per_step": round(dt_shuf * 1e3 / args.steps, 3),
-                "note": "the same box with its atoms in random order"}
-    # the same step with fixed-order force sums (opt-in: forces the same bits on every run)
-    sp.ordered_forces = True
-    log("timing", ":".join(chosen), "with ordered force sums")
-    r_ord = time_form(chosen)
-    F1 = sp.descriptor_and_vjp(sys_dev, w)[1]
-    F2 = sp.descriptor_and_vjp(sys_dev, w)[1]
-    sp.ordered_forces = False
-    F0 = sp.descriptor_and_vjp(sys_dev, w)[1]
-    ordered = {"value": round(r_ord["value"], 1), "ms_per_step": round(r_ord["ms_per_step"], 3),
-               "kernel_ms": {k: round(v, 4) for k, v in r_ord["kernel_ms"].items()},
-               "forces_bitwise_equal_on_repeat": bool(torch.equal(F1, F2)),
-               "max_rel_diff_vs_atomic_sums": float((F1 - F0).abs().max() / F0.abs().max())}
-    del F0, F1, F2
+    # beside the chosen step (skipped with --no-beside): the atoms in random order, and the
+    # fixed-order force sums
+    shuffled = ordered = None
+    if not args.no_beside:
+        # the same step with the atoms in random order (the edge pass groups centers along a
+        # Morton curve of their coordinates, so its speed should not depend on the order)
+        perm = np.random.default_rng(17).permutation(n_env)
+        sys_shuf = dict(system, positions=torch.tensor(np.asarray(system["positions"])[perm], device=dev),
+                        atomic_numbers=np.asarray(system["atomic_numbers"])[perm])
+        for _ in range(max(args.warmup, 1)):
+            d, F = sp.descriptor_and_vjp(sys_shuf, w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            d, F = sp.descriptor_and_vjp(sys_shuf, w)
+        torch.cuda.synchronize()
+        dt_shuf = time.perf_counter() - t0
+        del d, F, sys_shuf
+        shuffled = {"value": round(n_env * args.steps / dt_shuf, 1), "ms_per_step": round(dt_shuf * 1e3 / args.steps, 3),
+                    "note": "the same box with its atoms in random order"}
+        # the same step with fixed-order force sums (opt-in: forces the same bits on every run)
+        sp.ordered_forces = True
+        log("timing", ":".join(chosen), "with ordered force sums")
+        r_ord = time_form(chosen)
+        F1 = sp.descriptor_and_vjp(sys_dev, w)[1]
+        F2 = sp.descriptor_and_vjp(sys_dev, w)[1]
+        sp.ordered_forces = False
+        F0 = sp.descriptor_and_vjp(sys_dev, w)[1]
+        ordered = {"value": round(r_ord["value"], 1), "ms_per_step": round(r_ord["ms_per_step"], 3),
+                   "kernel_ms": {k: round(v, 4) for k, v in r_ord["kernel_ms"].items()},
+                   "forces_bitwise_equal_on_repeat": bool(torch.equal(F1, F2)),
+                   "max_rel_diff_vs_atomic_sums": float((F1 - F0).abs().max() / F0.abs().max())}
+        del F0, F1, F2
     nl = soap_for(*chosen)._neighbors(soap_for(*chosen)._context(sys_dev, None), drop_self=False)
     kbar = nl.n_edges / n_env
     dom = max(("coefficients", "vjp_edges"), key=lambda k: kern.get(k, 0.0))
@@ -364,8 +368,7 @@ def bench_adjoint(args):
                      "flops_per_launch": fl, "avg_kernel_ms": round(kern[dom], 4)},
         "kernel_ms": {k: round(v, 4) for k, v in kern.items()},
         "api_host_positions": api_host,
-        "shuffled_atoms": shuffled,
-        "ordered_force_sums": ordered,
+        **({"shuffled_atoms": shuffled, "ordered_force_sums": ordered} if shuffled is not None else {}),
         "forms": {form_name(f): {"value": round(r["value"], 1), "ms_per_step": round(r["ms_per_step"], 3),
                                  **{k: r[k] for k in ("desc_max_col_rel_err", "force_max_rel_err") if k in r},
                                  "kernel_ms": {k: round(v, 4) for k, v in r["kernel_ms"].items()}}
