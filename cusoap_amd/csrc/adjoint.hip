@@ -1313,8 +1313,9 @@ int side_stream(hipStream_t* out) {
 }
 
 // the centers' Morton order, in the workspace after the dL/dc rows (see vjp_ws_bytes), on
-// the side stream after the work already on st; *ready is recorded when it is done (the
-// caller waits on it, then destroys it)
+// the side stream after the work already on st; *ready is recorded after whatever was
+// enqueued there, also when a launch failed (the caller joins st to it on every path
+// before returning, see join_side, so no side work outlives the call's workspace)
 int morton_order(const cusoap_basis_t* b, const double* centers, int n_centers, void* ws, int32_t** order,
                  hipEvent_t* ready, hipStream_t st) {
     char* p = (char*)ws + vjp_core_ws_bytes(b, n_centers);
@@ -1329,11 +1330,18 @@ int morton_order(const cusoap_basis_t* b, const double* centers, int n_centers, 
     const hipError_t we = hipStreamWaitEvent(side, fork, 0);
     (void)hipEventDestroy(fork);             // released once recorded work completes
     if (we != hipSuccess) return fail(CUSOAP_ERR_HIP, std::string("vjp: stream wait: ") + hipGetErrorString(we));
-    if ((rc = center_order(centers, n_centers, p + align_up(sizeof(int32_t) * (size_t)n_centers, 256), *order, side)))
-        return rc;
+    rc = center_order(centers, n_centers, p + align_up(sizeof(int32_t) * (size_t)n_centers, 256), *order, side);
     CUSOAP_HIP(hipEventCreateWithFlags(ready, hipEventDisableTiming));
     CUSOAP_HIP(hipEventRecord(*ready, side));
-    return CUSOAP_OK;
+    return rc;
+}
+
+// st waits for the side stream's order work (a second wait after the edge pass's is
+// harmless), then the event is released
+void join_side(hipEvent_t ready, hipStream_t st) {
+    if (!ready) return;
+    (void)hipStreamWaitEvent(st, ready, 0);
+    (void)hipEventDestroy(ready);
 }
 
 int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int n_atoms, const int32_t* atom_sp,
@@ -1349,11 +1357,14 @@ int vjp_impl(const cusoap_basis_t* b, int mode, int prec, const double* pos, int
     if (n_centers == 0) return CUSOAP_OK;
     int32_t* order = nullptr;
     hipEvent_t ready = nullptr;
-    if ((rc = morton_order(b, centers, n_centers, ws, &order, &ready, st))) return rc;
+    if ((rc = morton_order(b, centers, n_centers, ws, &order, &ready, st))) {
+        join_side(ready, st);
+        return rc;
+    }
     rc = vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
                   offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, order, ready,
                   grad, VjpOrder{nullptr, nullptr, nullptr, 0}, st);
-    if (ready) (void)hipEventDestroy(ready);
+    join_side(ready, st);
     return rc;
 }
 
@@ -1399,13 +1410,16 @@ int vjp_ordered_impl(const cusoap_basis_t* b, int mode, int prec, const double* 
     if (n_centers == 0) return CUSOAP_OK;
     int32_t* order = nullptr;
     hipEvent_t ready = nullptr;
-    if ((rc = morton_order(b, centers, n_centers, ws, &order, &ready, st))) return rc;
+    if ((rc = morton_order(b, centers, n_centers, ws, &order, &ready, st))) {
+        join_side(ready, st);
+        return rc;
+    }
     char* p = (char*)ws + base;
     const VjpOrder ord{rev, p + order_center_bytes(n_centers), (double*)p, n_edges};
     rc = vjp_core(b, mode, prec, pos, n_atoms, atom_sp, shifts, centers, n_centers, center_attach, center_species,
                   offsets, nbr, coeffs, coeffs_dtype, pair_feature, gd, gd_dtype, gd_broadcast, n_feat, ws, order, ready,
                   grad, ord, st);
-    if (ready) (void)hipEventDestroy(ready);
+    join_side(ready, st);
     if (rc) return rc;
     prof_begin("vjp_gather", st);
     if (prec == CUSOAP_PREC_F64)
