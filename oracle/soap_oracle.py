@@ -12,10 +12,12 @@ stated per function.  The one deliberate substitution: the reference takes its
 real spherical / solid harmonics from the external ``sphericart`` library
 (SG:675-686); here they come from our own Cartesian recurrence
 (:func:`solid_harmonics`), pinned by the closed forms of the real-harmonic
-table for l<=3, orthonormality by quadrature, the addition theorem and finite
-differences.  For l>=2 the sign/m-order convention that sphericart uses is
-not checked against sphericart itself (it is not installed): cc features and
-cc Jacobians for l>=2 are "parity unpinned" with respect to sphericart.
+table for l<=3, by scipy's complex harmonics converted to the real basis
+without the Condon-Shortley phase (the convention sphericart documents) for
+every degree up to 9 (sign- and m-order-sensitive), by orthonormality by
+quadrature, the addition theorem and finite differences.  sphericart itself is
+not installed, so the identity with its implementation rests on that
+documented convention.
 
 Neighbour-list distances use the reference's formula
 d^2 = (|c|^2 + |n|^2) - 2 c.n (SG:1091-1113), with the sums written out in a
