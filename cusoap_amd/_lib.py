@@ -60,7 +60,8 @@ _SIGS = {
     "cusoap_vjp_ordered_workspace_size": [C.POINTER(Basis), i32, i32, i64, C.POINTER(sz)],
     "cusoap_vjp": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32, i32,
                    i32, p, sz, p, p],
-    "cusoap_edge_reverse": [p, i32, p, i32, p, i32, p, p, p, p],
+    "cusoap_edge_reverse_workspace_size": [i32, C.POINTER(sz)],
+    "cusoap_edge_reverse": [p, i32, p, i32, p, i32, p, p, p, sz, p, p],
     "cusoap_vjp_ordered": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i64, p, i32, p, p, i32,
                            i32, i32, p, sz, p, p],
     "cusoap_vjp_noalloc": [C.POINTER(Basis), i32, i32, p, i32, p, p, i32, p, i32, p, p, p, p, p, i32, p, p, i32,

@@ -1125,23 +1125,46 @@ __global__ void fill_rev_kernel(const int64_t* __restrict__ offsets, int n_cente
         rev[i] = -1;
 }
 
+// Per-center search block of the reverse-edge pass, 16 int32 (64 bytes): range start,
+// length, 1 if the center is its own atom (centers[c] == positions[c] bitwise), then the
+// keys at positions floor(k len / 8), k = 0..7, of its sorted range (INT_MAX past the end)
+constexpr int kRevBlock = 16;
+
+__global__ void reverse_index_kernel(const double* __restrict__ pos, int n_atoms, const double* __restrict__ centers,
+                                     int n_centers, const int64_t* __restrict__ offsets,
+                                     const int32_t* __restrict__ nbr, int32_t* __restrict__ index) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_centers) return;
+    const int64_t s0 = offsets[c];
+    const int len = (int)(offsets[c + 1] - s0);
+    int32_t* blk = index + (size_t)kRevBlock * c;
+    blk[0] = (int32_t)s0;
+    blk[1] = len;
+    blk[2] = (c < n_atoms && same_point(centers + 3 * (size_t)c, pos + 3 * (size_t)c)) ? 1 : 0;
+    blk[3] = 0;
+    for (int k = 0; k < 8; ++k) blk[4 + k] = len > 0 ? nbr[s0 + (int64_t)k * len / 8] : INT_MAX;
+    for (int k = 12; k < kRevBlock; ++k) blk[k] = 0;
+}
+
 // rev for the edges of center c (one wavefront per center; consecutive workgroups go to
 // different XCDs, each XCD taking a contiguous range of centers, whose neighbours' ranges
 // then share its L2): the edge c -> a and its reverse a -> c (the image with the opposite
 // shift) are linked both ways by the one of the two with c < a (c == a: the image with the
-// lower index), which finds the other by binary search in a's range.  rev stays -1 (set
-// beforehand) when the reverse is missing or the lists are not the atoms' own
+// lower index), which finds the other in a's range: a's search block (one read) gives the
+// eighth of the range that can hold the key, then a binary search within it.  rev stays -1
+// (set beforehand) when the reverse is missing or the lists are not the atoms' own
 // (centers[c] != positions[c] bitwise).  The image with the opposite shift mirrors img in
 // a -n..n grid of shifts; that guess is checked exactly, then every image is tried.
-__global__ __launch_bounds__(256) void edge_reverse_kernel(const double* __restrict__ pos, int n_atoms,
-                                                           const double* __restrict__ shifts, int n_img,
-                                                           const double* __restrict__ centers, int n_centers,
-                                                           int per_xcd, const int64_t* __restrict__ offsets,
-                                                           const int32_t* __restrict__ nbr, int32_t* __restrict__ rev) {
+__global__ __launch_bounds__(256) void edge_reverse_kernel(int n_atoms, const double* __restrict__ shifts, int n_img,
+                                                           int n_centers, int per_xcd,
+                                                           const int64_t* __restrict__ offsets,
+                                                           const int32_t* __restrict__ nbr,
+                                                           const int32_t* __restrict__ index,
+                                                           int32_t* __restrict__ rev) {
     const int lane = threadIdx.x & 63;
     const int c = ((int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3)) * 4 + (threadIdx.x >> 6);
     if (c >= n_centers || c >= n_atoms) return;
-    if (!same_point(centers + 3 * (size_t)c, pos + 3 * (size_t)c)) return;
+    if (!index[(size_t)kRevBlock * c + 2]) return;
     const double inv_na = 1.0 / (double)n_atoms;
     for (int64_t e = offsets[c] + lane; e < offsets[c + 1]; e += 64) {
         int img, a;
@@ -1162,15 +1185,23 @@ __global__ __launch_bounds__(256) void edge_reverse_kernel(const double* __restr
             rev[e] = (int32_t)e;
             continue;
         }
-        if (!same_point(centers + 3 * (size_t)a, pos + 3 * (size_t)a)) continue;
+        const int4* blk = (const int4*)(index + (size_t)kRevBlock * a);
+        const int4 h = blk[0], s0 = blk[1], s1 = blk[2];
+        if (!h.z) continue;
         const int key = t * n_atoms + c;
-        int64_t lo = offsets[a], hi = offsets[a + 1];
+        // the last sample <= key: its eighth of the range holds the key if the range does
+        const int j = (s0.x <= key) + (s0.y <= key) + (s0.z <= key) + (s0.w <= key) + (s1.x <= key) +
+                      (s1.y <= key) + (s1.z <= key) + (s1.w <= key) - 1;
+        if (j < 0) continue;
+        int lo = h.x + (int)((int64_t)j * h.y / 8);
+        const int end = h.x + (int)((int64_t)(j + 1) * h.y / 8);
+        int hi = end;
         while (lo < hi) {
-            const int64_t mid = lo + ((hi - lo) >> 1);
+            const int mid = lo + ((hi - lo) >> 1);
             if (nbr[mid] < key) lo = mid + 1;
             else hi = mid;
         }
-        if (lo < offsets[a + 1] && nbr[lo] == key) {
+        if (lo < end && nbr[lo] == key) {
             rev[e] = (int32_t)lo;
             rev[lo] = (int32_t)e;
         }
@@ -1374,19 +1405,28 @@ size_t vjp_ordered_ws_bytes(const cusoap_basis_t* b, int prec, int n_centers, in
     return vjp_ws_bytes(b, n_centers) + order_center_bytes(n_centers) + align_up(order_edge_bytes(prec) * e, 256);
 }
 
+size_t edge_reverse_ws_bytes(int n_centers) {
+    return align_up(sizeof(int32_t) * kRevBlock * (size_t)std::max(n_centers, 1), 256);
+}
+
 int edge_reverse_impl(const double* pos, int n_atoms, const double* shifts, int n_img, const double* centers,
-                      int n_centers, const int64_t* offsets, const int32_t* nbr, int32_t* rev, hipStream_t st) {
+                      int n_centers, const int64_t* offsets, const int32_t* nbr, void* ws, size_t ws_bytes,
+                      int32_t* rev, hipStream_t st) {
     if (n_atoms < 1 || n_img < 1 || n_centers < 0) return fail(CUSOAP_ERR_ARG, "edge_reverse: bad sizes");
-    if (!pos || !shifts || !centers || !offsets || !nbr || !rev) return fail(CUSOAP_ERR_ARG, "edge_reverse: null argument");
+    if (!pos || !shifts || !centers || !offsets || !nbr || !rev || !ws)
+        return fail(CUSOAP_ERR_ARG, "edge_reverse: null argument");
+    if (ws_bytes < edge_reverse_ws_bytes(n_centers)) return fail(CUSOAP_ERR_WORKSPACE, "edge_reverse: workspace too small");
     if (n_centers == 0) return CUSOAP_OK;
+    int32_t* index = (int32_t*)ws;
     // the edge count is read on the device (offsets[n_centers]): -1 over the whole range
     // by a grid-stride fill bounded by the last offset
     prof_begin("edge_reverse", st);
     fill_rev_kernel<<<1024, 256, 0, st>>>(offsets, n_centers, rev);
     CUSOAP_LAUNCH_CHECK();
+    reverse_index_kernel<<<(n_centers + 255) / 256, 256, 0, st>>>(pos, n_atoms, centers, n_centers, offsets, nbr, index);
+    CUSOAP_LAUNCH_CHECK();
     const int groups = (n_centers + 3) / 4, per = (groups + 7) / 8;
-    edge_reverse_kernel<<<8 * per, 256, 0, st>>>(pos, n_atoms, shifts, n_img, centers, n_centers, per, offsets, nbr,
-                                                 rev);
+    edge_reverse_kernel<<<8 * per, 256, 0, st>>>(n_atoms, shifts, n_img, n_centers, per, offsets, nbr, index, rev);
     prof_end("edge_reverse", st);
     CUSOAP_LAUNCH_CHECK();
     return CUSOAP_OK;

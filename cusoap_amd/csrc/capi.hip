@@ -72,8 +72,9 @@ int jacobian_dense_impl(const cusoap_basis_t*, int, const double*, int, const in
                         const int32_t*, int, const uint32_t*, int, void*, size_t, void*, int, int, hipStream_t);
 size_t vjp_ws_bytes(const cusoap_basis_t*, int);
 size_t vjp_ordered_ws_bytes(const cusoap_basis_t*, int, int, int64_t);
+size_t edge_reverse_ws_bytes(int);
 int edge_reverse_impl(const double*, int, const double*, int, const double*, int, const int64_t*, const int32_t*,
-                      int32_t*, hipStream_t);
+                      void*, size_t, int32_t*, hipStream_t);
 int vjp_ordered_impl(const cusoap_basis_t*, int, int, const double*, int, const int32_t*, const double*, int,
                      const double*, int, const int32_t*, const int32_t*, const int64_t*, const int32_t*,
                      const int32_t*, int64_t, const void*, int, const int32_t*, const void*, int, int, int, void*,
@@ -228,10 +229,16 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision, con
                     gd_dtype, gd_broadcast, n_feat, ws, ws_bytes, grad_pos, (hipStream_t)stream);
 }
 
+int cusoap_edge_reverse_workspace_size(int32_t n_centers, size_t* bytes) {
+    if (!bytes || n_centers < 0) return fail(CUSOAP_ERR_ARG, "edge_reverse_workspace_size: invalid argument");
+    *bytes = edge_reverse_ws_bytes(n_centers);
+    return CUSOAP_OK;
+}
+
 int cusoap_edge_reverse(const double* positions, int32_t n_atoms, const double* shifts, int32_t n_img,
                         const double* centers, int32_t n_centers, const int64_t* offsets, const int32_t* nbr,
-                        int32_t* rev, void* stream) {
-    return edge_reverse_impl(positions, n_atoms, shifts, n_img, centers, n_centers, offsets, nbr, rev,
+                        void* ws, size_t ws_bytes, int32_t* rev, void* stream) {
+    return edge_reverse_impl(positions, n_atoms, shifts, n_img, centers, n_centers, offsets, nbr, ws, ws_bytes, rev,
                              (hipStream_t)stream);
 }
 

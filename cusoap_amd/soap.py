@@ -1152,10 +1152,14 @@ class SOAP:
         side = self._side_stream
         side.wait_stream(main)
         rev = torch.empty(nl.n_edges, dtype=torch.int32, device=self.device)
+        nbytes = C.c_size_t(0)
+        _lib.check(lib.cusoap_edge_reverse_workspace_size(x.n_centers, C.byref(nbytes)), "edge_reverse_workspace_size")
+        ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
         _lib.check(lib.cusoap_edge_reverse(x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
                                            x.centers.data_ptr(), x.n_centers, nl.offsets.data_ptr(),
-                                           nl.nbr.data_ptr(), rev.data_ptr(), side.cuda_stream), "edge_reverse")
-        for t in (rev, x.pos, x.shifts, x.centers, nl.offsets, nl.nbr):
+                                           nl.nbr.data_ptr(), ws.data_ptr(), ws.numel(), rev.data_ptr(),
+                                           side.cuda_stream), "edge_reverse")
+        for t in (rev, ws, x.pos, x.shifts, x.centers, nl.offsets, nl.nbr):
             t.record_stream(side)
         ev = torch.cuda.Event()
         ev.record(side)

@@ -255,6 +255,8 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
  * of the cutoff) or the lists are not the atoms' own (centers[c] or centers[a]
  * differs from the atom's position, bitwise).  Depends on the neighbour lists
  * only, so it may run on another stream while the coefficients are formed.
+ * Workspace: cusoap_edge_reverse_workspace_size bytes (a 64-byte search block
+ * per center).
  * cusoap_vjp_ordered: cusoap_vjp for every atom its own center (n_centers ==
  * n_atoms, centers[c] == positions[c]) with rev from cusoap_edge_reverse and
  * n_edges = offsets[n_centers]: each edge's contribution is stored at its own
@@ -262,9 +264,10 @@ int cusoap_vjp(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
  * contributions of its reverse edges, so the result is the same bits on every
  * run.  Edges with rev = -1 add atomically (as cusoap_vjp).  Workspace:
  * cusoap_vjp_ordered_workspace_size bytes. */
+int cusoap_edge_reverse_workspace_size(int32_t n_centers, size_t* bytes);
 int cusoap_edge_reverse(const double* positions, int32_t n_atoms, const double* shifts, int32_t n_img,
                         const double* centers, int32_t n_centers, const int64_t* offsets, const int32_t* nbr,
-                        int32_t* rev, void* stream);
+                        void* ws, size_t ws_bytes, int32_t* rev, void* stream);
 int cusoap_vjp_ordered_workspace_size(const cusoap_basis_t* basis, int32_t precision, int32_t n_centers,
                                       int64_t n_edges, size_t* bytes);
 int cusoap_vjp_ordered(const cusoap_basis_t* basis, int32_t mode, int32_t precision,
