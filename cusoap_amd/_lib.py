@@ -49,6 +49,8 @@ _SIGS = {
     "cusoap_nl_count": [C.POINTER(NLPlan), p, i32, p, i32, p, i32, f64, i32, p, sz, p,
                         C.POINTER(i64), C.POINTER(i32), p],
     "cusoap_nl_fill": [C.POINTER(NLPlan), p, i32, p, i32, p, i32, f64, i32, p, sz, p, i32, p, p],
+    "cusoap_nl_build": [p, i32, p, i32, p, i32, f64, i32, p, p, sz, p, p, i64, C.POINTER(NLPlan), C.POINTER(i64),
+                        C.POINTER(i32), p],
     "cusoap_coefficients": [C.POINTER(Basis), i32, p, i32, p, p, i32, p, i32, p, p, p, i32, p, p],
     "cusoap_power_spectrum": [C.POINTER(Basis), p, i32, i32, p, i32, p, i32, p],
     "cusoap_cc_projection": [C.POINTER(Basis), p, i32, i32, p, i32, p],

@@ -166,6 +166,22 @@ int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_
                         offsets, max_count, nbr, (hipStream_t)stream);
 }
 
+int cusoap_nl_build(const double* positions, int32_t n_atoms, const double* shifts, int32_t n_img,
+                    const double* centers, int32_t n_centers, double cutoff, int32_t drop_self, void* tmp, void* ws,
+                    size_t ws_bytes, int64_t* offsets, int32_t* nbr, int64_t nbr_capacity, cusoap_nl_plan_t* plan,
+                    int64_t* total_host, int32_t* max_count_host, void* stream) {
+    const hipStream_t st = (hipStream_t)stream;
+    if (!plan || !total_host || !max_count_host) return fail(CUSOAP_ERR_ARG, "nl_build: null argument");
+    int rc = nl_plan_impl(centers, n_centers, n_atoms, n_img, cutoff, tmp, plan, st);
+    if (rc) return rc;
+    if (ws_bytes < plan->ws_bytes) return fail(CUSOAP_ERR_WORKSPACE, "nl_build: workspace smaller than the plan's");
+    rc = nl_count_impl(plan, positions, n_atoms, shifts, n_img, centers, n_centers, cutoff, drop_self, ws, ws_bytes,
+                       offsets, total_host, max_count_host, st);
+    if (rc || *total_host > nbr_capacity) return rc;
+    return nl_fill_impl(plan, positions, n_atoms, shifts, n_img, centers, n_centers, cutoff, drop_self, ws, ws_bytes,
+                        offsets, *max_count_host, nbr, st);
+}
+
 int cusoap_coefficients(const cusoap_basis_t* basis, int32_t precision, const double* positions, int32_t n_atoms,
                         const int32_t* atom_species, const double* shifts, int32_t n_img, const double* centers,
                         int32_t n_centers, const int32_t* center_self_species, const int64_t* offsets,

@@ -66,14 +66,15 @@ __device__ __forceinline__ int cell_coord(double v, double o, double h, int n) {
 // --- workspace layout -------------------------------------------------------
 struct NLWs {
     int32_t* ext_cell;     // (n_ext)
-    int32_t* cell_count;   // (n_cells)
+    int32_t* cell_count;   // (n_cells + 1)
+    int32_t* cursor;       // (n_cells + 1) fill cursors of the scatter pass
     int64_t* cell_start;   // (n_cells + 1)
     double4* binned;       // (n_ext)  x, y, z, |p|^2
     int32_t* binned_ext;   // (n_ext)
     int32_t* counts;       // (n_centers)
     int32_t* sorted;       // (n_centers, kSortCap) sorted edges of centers with <= kSortCap edges
     int64_t* partials;     // scan partials
-    int32_t* max_count;    // (1)
+    int32_t* max_count;    // (4): [0] the largest edge count; bytes 8..15 the edge total (int64)
     size_t bytes;
 };
 
@@ -98,6 +99,7 @@ NLWs carve(void* base, int64_t n_ext, int32_t n_cells, int32_t n_centers) {
     };
     w.ext_cell = (int32_t*)take(sizeof(int32_t) * n_ext);
     w.cell_count = (int32_t*)take(sizeof(int32_t) * (n_cells + 1));
+    w.cursor = (int32_t*)take(sizeof(int32_t) * (n_cells + 1));
     w.cell_start = (int64_t*)take(sizeof(int64_t) * (n_cells + 1));
     w.binned = (double4*)take(sizeof(double4) * n_ext);
     w.binned_ext = (int32_t*)take(sizeof(int32_t) * n_ext);
@@ -341,9 +343,23 @@ __device__ __forceinline__ CandRows cand_rows(const Grid& g, const CenterCells& 
 
 // largest per-center count: one atomic per workgroup (a per-center atomic on one
 // address serialises, ~1 ms per 10^5 centers)
+// zero the cell counts, the scatter cursors and the count summary (one launch)
+__global__ void nl_zero_kernel(int32_t* __restrict__ cell_count, int32_t* __restrict__ cursor, int n_cells1,
+                               int32_t* __restrict__ summary) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_cells1; i += gridDim.x * blockDim.x) {
+        cell_count[i] = 0;
+        cursor[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) summary[threadIdx.x] = 0;
+}
+
+// the largest per-center edge count, and (block 0) the edge total offsets[n] beside it,
+// so that one 16-byte copy brings both to the host
 __global__ __launch_bounds__(256) void max_count_kernel(const int32_t* __restrict__ counts, int n,
+                                                        const int64_t* __restrict__ offsets,
                                                         int32_t* __restrict__ max_count) {
     __shared__ int red[4];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ((int64_t*)max_count)[1] = offsets[n];
     int m = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, counts[i]);
 #pragma unroll
@@ -583,8 +599,9 @@ int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_ato
         return fail(CUSOAP_ERR_ARG, "nl_count: plan does not match n_atoms * n_img");
     NLWs w = carve(ws, plan->n_ext, plan->n_cells, n_centers);
     if (ws_bytes < w.bytes) return fail(CUSOAP_ERR_WORKSPACE, "nl_count: workspace too small");
-    CUSOAP_HIP(hipMemsetAsync(w.cell_count, 0, sizeof(int32_t) * (plan->n_cells + 1), st));
-    CUSOAP_HIP(hipMemsetAsync(w.max_count, 0, sizeof(int32_t) * 4, st));
+    nl_zero_kernel<<<std::max(1, std::min(1024, (plan->n_cells + 256) / 256)), 256, 0, st>>>(
+        w.cell_count, w.cursor, plan->n_cells + 1, w.max_count);
+    CUSOAP_LAUNCH_CHECK();
     const int64_t n_ext = plan->n_ext;
     if (n_ext > 0) {
         bin_kernel<<<(unsigned)((n_ext + 255) / 256), 256, 0, st>>>(*plan, pos, n_atoms, shifts, n_ext, w.ext_cell,
@@ -593,10 +610,9 @@ int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_ato
     }
     int rc = scan_exclusive(w.cell_count, plan->n_cells, w.cell_start, w.partials, st);
     if (rc) return rc;
-    CUSOAP_HIP(hipMemsetAsync(w.cell_count, 0, sizeof(int32_t) * (plan->n_cells + 1), st));
     if (n_ext > 0) {
         scatter_kernel<<<(unsigned)((n_ext + 255) / 256), 256, 0, st>>>(pos, n_atoms, shifts, n_ext, w.ext_cell,
-                                                                        w.cell_start, w.cell_count, w.binned,
+                                                                        w.cell_start, w.cursor, w.binned,
                                                                         w.binned_ext);
         CUSOAP_LAUNCH_CHECK();
     }
@@ -606,19 +622,21 @@ int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_ato
         count_sort_kernel<kSortCap><<<(n_centers + 3) / 4, 256, 0, st>>>(*plan, centers, n_centers, cutoff2,
                                                                           drop_self, w.cell_start, w.binned,
                                                                           w.binned_ext, w.counts, w.sorted);
-        max_count_kernel<<<std::min(256, (n_centers + 255) / 256), 256, 0, st>>>(w.counts, n_centers, w.max_count);
         prof_end("nl_count", st);
         CUSOAP_LAUNCH_CHECK();
     }
     rc = scan_exclusive(w.counts, n_centers, offsets, w.partials, st);
     if (rc) return rc;
-    int64_t total = 0;
-    int32_t mx = 0;
-    CUSOAP_HIP(hipMemcpyAsync(&total, offsets + n_centers, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    CUSOAP_HIP(hipMemcpyAsync(&mx, w.max_count, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (n_centers > 0) {
+        max_count_kernel<<<std::min(256, (n_centers + 255) / 256), 256, 0, st>>>(w.counts, n_centers, offsets,
+                                                                                   w.max_count);
+        CUSOAP_LAUNCH_CHECK();
+    }
+    int64_t summary[2] = {0, 0};                 // [0] low 32 bits: max count, [1] edge total
+    CUSOAP_HIP(hipMemcpyAsync(summary, w.max_count, sizeof(summary), hipMemcpyDeviceToHost, st));
     CUSOAP_HIP(hipStreamSynchronize(st));
-    if (total_host) *total_host = total;
-    if (max_host) *max_host = mx;
+    if (total_host) *total_host = n_centers > 0 ? summary[1] : 0;
+    if (max_host) *max_host = (int32_t)(summary[0] & 0xffffffff);
     return CUSOAP_OK;
 }
 

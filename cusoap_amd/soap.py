@@ -274,6 +274,9 @@ class SOAP:
         self._basis_learned = None
         self._species_cache = None   # device tensors derived from (Z, cell, pbc) of the last system
         self._last_nl_backend = "hip-cell-list"
+        # neighbour-search buffer sizes (workspace bytes, edges) of the last search per
+        # (centers, atoms, images, cutoff, drop_self): lets one call build the next list
+        self._nl_sizes = {}
         if radial not in ("exact", "table", "learned", "chebyshev"):
             raise ValueError("radial must be 'exact', 'table', 'learned' or 'chebyshev'")
         if spectrum not in _SPECTRA:
@@ -698,21 +701,48 @@ class SOAP:
         cutoff = self._cutoff if cutoff is None else float(cutoff)
         plan = _lib.NLPlan()
         tmp = torch.empty(16384, dtype=torch.uint8, device=self.device)
-        _lib.check(lib.cusoap_nl_plan(x.centers.data_ptr(), x.n_centers, x.n_atoms, x.n_img, cutoff,
-                                      tmp.data_ptr(), C.byref(plan), st), "nl_plan")
-        ws = torch.empty(int(plan.ws_bytes), dtype=torch.uint8, device=self.device)
         offsets = torch.empty(x.n_centers + 1, dtype=torch.int64, device=self.device)
         total, mx = C.c_int64(0), C.c_int32(0)
-        _lib.check(lib.cusoap_nl_count(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
-                                       x.centers.data_ptr(), x.n_centers, cutoff, int(drop_self),
-                                       ws.data_ptr(), ws.numel(), offsets.data_ptr(), C.byref(total), C.byref(mx),
-                                       st), "nl_count")
+        key = (x.n_centers, x.n_atoms, x.n_img, cutoff, bool(drop_self))
+        sizes = self._nl_sizes.get(key)
+        ws = nbr = None
+        if sizes is not None:
+            # plan, count and fill in one call (no host work between them), with the
+            # buffers sized from the last search on this shape
+            ws = torch.empty(sizes[0], dtype=torch.uint8, device=self.device)
+            nbr = torch.empty(sizes[1], dtype=torch.int32, device=self.device)
+            rc = lib.cusoap_nl_build(x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img, x.centers.data_ptr(),
+                                     x.n_centers, cutoff, int(drop_self), tmp.data_ptr(), ws.data_ptr(), ws.numel(),
+                                     offsets.data_ptr(), nbr.data_ptr(), nbr.numel(), C.byref(plan), C.byref(total),
+                                     C.byref(mx), st)
+            if rc == _lib.OK and int(total.value) <= nbr.numel():
+                self._nl_sizes[key] = (sizes[0], max(sizes[1], self._nbr_room(int(total.value))))
+                return _NL(offsets, nbr, int(total.value), int(mx.value))
+            if rc not in (_lib.OK, _lib.ERR_WORKSPACE):
+                _lib.check(rc, "nl_build")
+            if rc == _lib.ERR_WORKSPACE:
+                ws = None                      # the plan is filled; count with a workspace of its size
+        else:
+            _lib.check(lib.cusoap_nl_plan(x.centers.data_ptr(), x.n_centers, x.n_atoms, x.n_img, cutoff,
+                                          tmp.data_ptr(), C.byref(plan), st), "nl_plan")
+        if ws is None:
+            ws = torch.empty(int(plan.ws_bytes), dtype=torch.uint8, device=self.device)
+            _lib.check(lib.cusoap_nl_count(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(),
+                                           x.n_img, x.centers.data_ptr(), x.n_centers, cutoff, int(drop_self),
+                                           ws.data_ptr(), ws.numel(), offsets.data_ptr(), C.byref(total),
+                                           C.byref(mx), st), "nl_count")
         nbr = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=self.device)
         _lib.check(lib.cusoap_nl_fill(C.byref(plan), x.pos.data_ptr(), x.n_atoms, x.shifts.data_ptr(), x.n_img,
                                       x.centers.data_ptr(), x.n_centers, cutoff, int(drop_self),
                                       ws.data_ptr(), ws.numel(), offsets.data_ptr(), mx.value, nbr.data_ptr(), st),
                    "nl_fill")
+        self._nl_sizes[key] = (int(plan.ws_bytes) + int(plan.ws_bytes) // 16, self._nbr_room(int(total.value)))
         return _NL(offsets, nbr, int(total.value), int(mx.value))
+
+    @staticmethod
+    def _nbr_room(n_edges: int) -> int:
+        """Edge capacity for the next search on the same shape: 1/8 more, at least 1024."""
+        return n_edges + n_edges // 8 + 1024
 
     def _coeff_dtype(self, precision: str) -> torch.dtype:
         return torch.float64 if precision == "float64" else torch.float32

@@ -151,6 +151,17 @@ int cusoap_nl_fill(const cusoap_nl_plan_t* plan, const double* positions, int32_
                    const double* shifts, int32_t n_img, const double* centers, int32_t n_centers,
                    double cutoff, int32_t drop_self, void* ws, size_t ws_bytes,
                    const int64_t* offsets, int32_t max_count, int32_t* nbr, void* stream);
+/* plan + count + fill in one call, for callers that can size the buffers ahead (from
+ * an earlier call on a similar system): no host work between the three stages.
+ * Returns CUSOAP_ERR_WORKSPACE when ws_bytes < plan->ws_bytes (plan filled, nothing
+ * counted: call again with that workspace, or continue with cusoap_nl_count).  When
+ * the edge total exceeds nbr_capacity the fill is skipped (the return is CUSOAP_OK
+ * with *total_host > nbr_capacity): complete it with cusoap_nl_fill on an nbr of
+ * *total_host entries and the same workspace. */
+int cusoap_nl_build(const double* positions, int32_t n_atoms, const double* shifts, int32_t n_img,
+                    const double* centers, int32_t n_centers, double cutoff, int32_t drop_self, void* tmp,
+                    void* ws, size_t ws_bytes, int64_t* offsets, int32_t* nbr, int64_t nbr_capacity,
+                    cusoap_nl_plan_t* plan, int64_t* total_host, int32_t* max_count_host, void* stream);
 
 /* ---- density coefficients (replaces _coefficients_gto / _coefficients_polynomial
  *      / _add_self_terms, soap_generator.py:1294-1435, 821-850, and the
