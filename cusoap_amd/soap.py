@@ -179,16 +179,24 @@ class SOAP:
     (float32 harmonics and accumulation, float64 radial functions) or
     "float32".  Output tensors are ``dtype`` as in the reference.
 
-    Extra keyword ``radial``: "exact" (default), "table" or "learned".
+    Extra keyword ``radial``: "exact" (default), "table", "chebyshev" or "learned".
     "table" tabulates the exact float64 GTO radial functions and slopes once
     (``table_intervals`` cubic-Hermite intervals on [0, cutoff]).  The coefficient
     and adjoint kernels then interpolate per edge, with no exponentials and no
-    Loewdin contraction per edge.  "learned" trains (or loads,
-    ``surrogate=<checkpoint path>``) per-species-pair networks for the radial
-    functions on the device and tabulates the gated model the same way
+    Loewdin contraction per edge.  "chebyshev" (float precisions, unweighted GTO,
+    n_max <= 8, l_max <= 6, at most 4 species) expands the exact functions in
+    piecewise Chebyshev series so that the coefficient stage runs on the matrix
+    cores (cusoap_amd.chebyshev); the adjoint uses the table.  "learned" trains (or
+    loads, ``surrogate=<checkpoint path>``) per-species-pair networks for the radial
+    functions on the device and feeds the gated model to both stages the same ways
     (cusoap_amd.surrogate).  ``surrogate`` may also be a RadialSurrogate or a dict
     of its constructor and train() options.  The dense Jacobian always uses the
     exact radial functions.
+
+    Attribute ``ordered_forces`` (default False): when True and every atom is a
+    center, ``vjp`` / ``descriptor_and_vjp`` sum each atom's force in a fixed order
+    (cusoap_edge_reverse + cusoap_vjp_ordered), so the forces are the same bits on
+    every run; otherwise they are summed with float64 atomics.
 
     Extra keyword ``spectrum``: arithmetic of the power-spectrum contraction.
     "exact" (default) uses the vector units in the coefficient precision.
