@@ -341,8 +341,6 @@ __device__ __forceinline__ CandRows cand_rows(const Grid& g, const CenterCells& 
     return cr;
 }
 
-// largest per-center count: one atomic per workgroup (a per-center atomic on one
-// address serialises, ~1 ms per 10^5 centers)
 // zero the cell counts, the scatter cursors and the count summary (one launch)
 __global__ void nl_zero_kernel(int32_t* __restrict__ cell_count, int32_t* __restrict__ cursor, int n_cells1,
                                int32_t* __restrict__ summary) {
@@ -353,8 +351,9 @@ __global__ void nl_zero_kernel(int32_t* __restrict__ cell_count, int32_t* __rest
     if (blockIdx.x == 0 && threadIdx.x < 4) summary[threadIdx.x] = 0;
 }
 
-// the largest per-center edge count, and (block 0) the edge total offsets[n] beside it,
-// so that one 16-byte copy brings both to the host
+// the largest per-center edge count, one atomic per workgroup (a per-center atomic on one
+// address serialises, ~1 ms per 10^5 centers), and (block 0) the edge total offsets[n]
+// beside it, so that one 16-byte copy brings both to the host
 __global__ __launch_bounds__(256) void max_count_kernel(const int32_t* __restrict__ counts, int n,
                                                         const int64_t* __restrict__ offsets,
                                                         int32_t* __restrict__ max_count) {
