@@ -251,6 +251,15 @@ def bench_adjoint(args):
     names = ("coefficients", "vjp_edges", "vjp_adjoint", "edge_reverse", "vjp_gather", "center_order", "nl_count",
              "nl_fill", "power_spectrum")
 
+    # full-box reference: the float64 GPU form (within 1e-7 / 1e-10 of the float64 oracle on
+    # its first centers), against which every form's descriptors and forces are checked on
+    # all atoms as well
+    log("float64 GPU reference on the full box")
+    d64_full, F64_full = soap_for("float64", "exact", "exact").descriptor_and_vjp(sys_dev, w)
+    d64_full = d64_full.double()
+    d64_scale = d64_full.abs().amax(0).clamp_min(1e-300)
+    F64_scale = F64_full.abs().max()
+
     def time_form(form):
         sp = soap_for(*form)
         for _ in range(args.warmup):
@@ -278,6 +287,9 @@ def bench_adjoint(args):
             res["desc_max_col_rel_err"] = float((np.abs(dd - desc_ref).max(0) / np.where(sc > 0, sc, 1.0)).max())
             _, Fs = sp.descriptor_and_vjp(sys_dev, w, centers=list(range(n_cpu)))
             res["force_max_rel_err"] = float(np.abs(Fs.cpu().numpy() - F_ref).max() / np.abs(F_ref).max())
+        res["full_box_vs_gpu_float64"] = {
+            "desc_max_col_rel_err": float(((d.double() - d64_full).abs().amax(0) / d64_scale).max()),
+            "force_max_rel_err": float((F - F64_full).abs().max() / F64_scale)}
         del d, F
         return res
 
@@ -370,14 +382,16 @@ def bench_adjoint(args):
         "api_host_positions": api_host,
         **({"shuffled_atoms": shuffled, "ordered_force_sums": ordered} if shuffled is not None else {}),
         "forms": {form_name(f): {"value": round(r["value"], 1), "ms_per_step": round(r["ms_per_step"], 3),
-                                 **{k: r[k] for k in ("desc_max_col_rel_err", "force_max_rel_err") if k in r},
+                                 **{k: r[k] for k in ("desc_max_col_rel_err", "force_max_rel_err",
+                                                      "full_box_vs_gpu_float64") if k in r},
                                  "kernel_ms": {k: round(v, 4) for k, v in r["kernel_ms"].items()}}
                   for f, r in results.items()},
     }
     if ("float32", "exact", "exact") in results:
         out["speedup_vs_plain_float32"] = round(main_res["value"] / results[("float32", "exact", "exact")]["value"], 3)
     if cpu is not None:
-        out["error_reference"] = f"float64 CPU oracle on the first {n_cpu} centers"
+        out["error_reference"] = (f"float64 CPU oracle on the first {n_cpu} centers; full_box_vs_gpu_float64: "
+                                  f"the float64 GPU form on all {n_env} atoms")
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 

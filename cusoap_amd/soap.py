@@ -181,7 +181,10 @@ class SOAP:
 
     Extra keyword ``radial``: "exact" (default), "table", "chebyshev" or "learned".
     "table" tabulates the exact float64 GTO radial functions and slopes once
-    (``table_intervals`` cubic-Hermite intervals on [0, cutoff]).  The coefficient
+    (``table_intervals`` cubic-Hermite intervals on [0, cutoff]; by default 256 for the
+    learned model and 512 for the exact functions in the float precisions, whose float32
+    slopes get noisier on finer tables, and 1024 in float64:
+    cusoap_amd.surrogate.default_table_intervals).  The coefficient
     and adjoint kernels then interpolate per edge, with no exponentials and no
     Loewdin contraction per edge.  "chebyshev" (float precisions, unweighted GTO,
     n_max <= 8, l_max <= 6, at most 4 species) expands the exact functions in
@@ -208,7 +211,7 @@ class SOAP:
     def __init__(self, r_cut=None, n_max=None, l_max=None, sigma=1.0, rbf="gto", weighting=None,
                  crossover=True, average="off", species=None, periodic=False, sparse=False,
                  dtype="float32", device=None, quad_n=100, max_num_neighbors=None,
-                 precision="float64", radial="exact", surrogate=None, table_intervals=1024, spectrum="exact",
+                 precision="float64", radial="exact", surrogate=None, table_intervals=None, spectrum="exact",
                  _betas=None):
         if n_max is None or l_max is None:
             raise ValueError("n_max and l_max must be provided.")
@@ -299,6 +302,9 @@ class SOAP:
         self.surrogate = None
         self._rad_table = None
         self._rad_cheb = None
+        if table_intervals is None:
+            from .surrogate import default_table_intervals
+            table_intervals = default_table_intervals(radial, precision)
         if radial == "learned":
             self._init_learned_radial(surrogate, int(table_intervals))
         elif radial == "table":

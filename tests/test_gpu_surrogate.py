@@ -204,8 +204,8 @@ def test_exact_table_interpolation_matches_oracle(n_max, l_max):
     w = np.random.default_rng(5).normal(size=o.get_number_of_features())
     Fr = o.vjp(s, w[None, :].repeat(len(s["positions"]), 0))
     ref = o.create(s)
-    # forces: the cubic-Hermite interpolation of the 1024-interval table itself reaches
-    # ~1e-5 of max|F| on the smallest basis (n 4, l 2), in float64 as in mixed
+    # forces: the cubic-Hermite interpolation of the table itself (1024 intervals in
+    # float64, 512 in mixed) reaches ~1e-5 of max|F| on the smallest basis (n 4, l 2)
     for prec, tol in (("float64", (1e-6, 2e-5)), ("mixed", (1e-5, 2e-5))):
         g = SOAP(device=DEV, dtype="float64", precision=prec, radial="table", _betas=o.betas, **p)
         d, F = g.descriptor_and_vjp(s, torch.tensor(w))
