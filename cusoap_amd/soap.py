@@ -295,7 +295,7 @@ class SOAP:
         self.spectrum = spectrum
         self.radial = radial
         # True: fixed-order force sums in the adjoint when every atom is a center (forces
-        # the same bits on every run; cusoap_edge_reverse + cusoap_vjp_ordered, ~0.5 ms more
+        # the same bits on every run; cusoap_edge_reverse + cusoap_vjp_ordered, ~0.35 ms more
         # on a 98,304-atom water box step); False (default): float64 atomics
         self.ordered_forces = False
         self._side_stream = None
