@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstring>
 
 #include <type_traits>
 
@@ -545,6 +546,22 @@ __global__ __launch_bounds__(256) void copy_sorted_kernel(int n_centers, const i
     for (int i = lane; i < cnt; i += 64) nbr[b + i] = src[i];
 }
 
+// Pinned host landing space for the plan's bounding box and the count's summary (64 bytes
+// per host thread, allocated on first use and kept until the process ends): a device-to-host
+// copy into pageable memory is staged through a runtime buffer, into pinned memory it is
+// one transfer that the following synchronisation waits for
+int host_landing(double** out) {
+    static thread_local double* pinned = nullptr;
+    if (!pinned) {
+        void* p = nullptr;
+        const hipError_t e = hipHostMalloc(&p, 64, hipHostMallocPortable);
+        if (e != hipSuccess) return fail(CUSOAP_ERR_HIP, std::string("nl: pinned host buffer: ") + hipGetErrorString(e));
+        pinned = (double*)p;
+    }
+    *out = pinned;
+    return CUSOAP_OK;
+}
+
 }  // namespace
 
 int nl_plan_impl(const double* centers, int32_t n_centers, int32_t n_atoms, int32_t n_img, double cutoff,
@@ -555,12 +572,15 @@ int nl_plan_impl(const double* centers, int32_t n_centers, int32_t n_atoms, int3
     if (n_ext >= INT_MAX) return fail(CUSOAP_ERR_UNSUPPORTED, "nl_plan: n_images * n_atoms must be < 2^31");
     double bb[6] = {0, 0, 0, 0, 0, 0};
     if (n_centers > 0) {
+        double* land = nullptr;
+        if (int rc = host_landing(&land)) return rc;
         const int nblk = std::min(256, std::max(1, (n_centers + 255) / 256));
         bbox_partial_kernel<<<nblk, 256, 0, st>>>(centers, n_centers, (double*)tmp);
         bbox_final_kernel<<<1, 64, 0, st>>>((double*)tmp, nblk);
         CUSOAP_LAUNCH_CHECK();
-        CUSOAP_HIP(hipMemcpyAsync(bb, tmp, sizeof(bb), hipMemcpyDeviceToHost, st));
+        CUSOAP_HIP(hipMemcpyAsync(land, tmp, sizeof(bb), hipMemcpyDeviceToHost, st));
         CUSOAP_HIP(hipStreamSynchronize(st));
+        std::copy(land, land + 6, bb);
     }
     const double margin = cutoff * (1.0 + 1e-9) + 1e-6;
     int64_t dims[3];
@@ -632,8 +652,11 @@ int nl_count_impl(const cusoap_nl_plan_t* plan, const double* pos, int32_t n_ato
         CUSOAP_LAUNCH_CHECK();
     }
     int64_t summary[2] = {0, 0};                 // [0] low 32 bits: max count, [1] edge total
-    CUSOAP_HIP(hipMemcpyAsync(summary, w.max_count, sizeof(summary), hipMemcpyDeviceToHost, st));
+    double* land = nullptr;
+    if (int rc = host_landing(&land)) return rc;
+    CUSOAP_HIP(hipMemcpyAsync(land, w.max_count, sizeof(summary), hipMemcpyDeviceToHost, st));
     CUSOAP_HIP(hipStreamSynchronize(st));
+    std::memcpy(summary, land, sizeof(summary));
     if (total_host) *total_host = n_centers > 0 ? summary[1] : 0;
     if (max_host) *max_host = (int32_t)(summary[0] & 0xffffffff);
     return CUSOAP_OK;
